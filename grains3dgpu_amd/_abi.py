@@ -1,0 +1,64 @@
+"""ctypes mirror of the structs and constants of include/grains_hip.h.
+
+The same layouts are used by the HIP library and by the FP64 CPU oracle,
+so one scene description drives both.
+"""
+import ctypes as C
+
+OK = 0
+EINVAL, ENOMEM, EDEVICE, ESTATE, EOVERFLOW, EUNSUPPORTED = -1, -2, -3, -4, -5, -6
+
+# ConvexType numbering (Grains/Geometry/include/Convex.hh:7-16)
+SPHERE, BOX, CYLINDER, CONE, SUPERQUADRIC, POLYHEDRON, RECTANGLE = range(7)
+
+FP64, FP32_MIXED = 0, 1
+REF_QUIRKS = 0x1
+NESTEROV = 0x2
+
+PAIRS_CANDIDATE, PAIRS_CONTACT = 0, 1
+
+
+class Shape(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("material", C.c_int32),
+                ("dims", C.c_double * 5), ("crust", C.c_double),
+                ("density", C.c_double), ("n_vertices", C.c_int32),
+                ("vertex_offset", C.c_int32)]
+
+
+class Hooke(C.Structure):
+    _fields_ = [("kn", C.c_double), ("en", C.c_double), ("etat", C.c_double),
+                ("muc", C.c_double), ("kr", C.c_double)]
+
+
+class Sim(C.Structure):
+    _fields_ = [("domain_min", C.c_double * 3), ("domain_max", C.c_double * 3),
+                ("cell_size", C.c_double), ("cell_size_factor", C.c_double),
+                ("gravity", C.c_double * 3), ("periodic", C.c_int32),
+                ("contact_table", C.c_int32)]
+
+
+class Config(C.Structure):
+    _fields_ = [("dt", C.c_double), ("precision", C.c_int32),
+                ("flags", C.c_uint32), ("gjk_rel_tol", C.c_double),
+                ("max_neighbors", C.c_int32), ("block_size", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("candidate_pairs", C.c_uint64),
+                ("gjk_calls", C.c_uint64), ("gjk_iterations", C.c_uint64),
+                ("contacts", C.c_uint64), ("wall_contacts", C.c_uint64),
+                ("out_of_domain", C.c_uint64), ("deep_contacts", C.c_uint64),
+                ("fp64_refines", C.c_uint64), ("neighbor_overflow", C.c_uint64),
+                ("max_candidates", C.c_uint32), ("sort_passes", C.c_uint32),
+                ("elapsed_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def dptr(a):
+    """Pointer to a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)

@@ -1,0 +1,278 @@
+"""Scene description shared by the HIP library and the CPU oracle, plus the
+seeded generators of the benchmark workloads.
+
+A `Scene` holds everything `grains_set_*` takes (include/grains_hip.h); it
+plays the role of the reference's XML case file (Grains/Base/src/Grains.cpp:
+91-401), already parsed.  Units are SI, as in the reference.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+
+
+@dataclass
+class Scene:
+    shapes: list                       # list of dict(kind, material, dims, crust, density, n_vertices, vertex_offset)
+    tables: np.ndarray                 # (n_tables * n_pairs, 5): kn, en, etat, muc, kr
+    n_materials: int
+    sims: list                         # list of dict (grains_sim_t fields)
+    particle_shape: np.ndarray         # (n,) uint32
+    pos: np.ndarray                    # (n, 3)
+    quat: np.ndarray                   # (n, 4) x, y, z, w
+    vel: np.ndarray                    # (n, 3)
+    omega: np.ndarray                  # (n, 3)
+    particle_sim: np.ndarray | None = None
+    obstacle_shape: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    obstacle_sim: np.ndarray | None = None
+    obstacle_tr: np.ndarray = field(default_factory=lambda: np.zeros((0, 12)))
+    vertices: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    dt: float = 1e-6
+    name: str = "scene"
+    notes: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+    # -- ctypes views ------------------------------------------------------
+    def shape_array(self):
+        arr = (A.Shape * max(1, len(self.shapes)))()
+        for i, s in enumerate(self.shapes):
+            arr[i].kind = s["kind"]
+            arr[i].material = s.get("material", 0)
+            d = list(s["dims"]) + [0.0] * (5 - len(s["dims"]))
+            for k in range(5):
+                arr[i].dims[k] = d[k]
+            arr[i].crust = s.get("crust", 0.0)
+            arr[i].density = s.get("density", 0.0)
+            arr[i].n_vertices = s.get("n_vertices", 0)
+            arr[i].vertex_offset = s.get("vertex_offset", 0)
+        return arr
+
+    def table_array(self):
+        t = np.ascontiguousarray(self.tables, dtype=np.float64)
+        arr = (A.Hooke * t.shape[0])()
+        for i in range(t.shape[0]):
+            arr[i].kn, arr[i].en, arr[i].etat, arr[i].muc, arr[i].kr = t[i]
+        return arr
+
+    def sim_array(self):
+        arr = (A.Sim * len(self.sims))()
+        for i, s in enumerate(self.sims):
+            for k in range(3):
+                arr[i].domain_min[k] = s["domain_min"][k]
+                arr[i].domain_max[k] = s["domain_max"][k]
+                arr[i].gravity[k] = s.get("gravity", (0.0, 0.0, -9.81))[k]
+            arr[i].cell_size = s.get("cell_size", 0.0)
+            arr[i].cell_size_factor = s.get("cell_size_factor", 1.0)
+            arr[i].periodic = s.get("periodic", 0)
+            arr[i].contact_table = s.get("contact_table", 0)
+        return arr
+
+    @property
+    def n_tables(self) -> int:
+        per = self.n_materials * (self.n_materials + 1) // 2
+        return int(self.tables.shape[0] // per)
+
+
+def load(backend, scene: Scene):
+    """Drive any backend exposing the set_* calls of include/grains_hip.h."""
+    backend.set_shapes(scene)
+    backend.set_contact_tables(scene)
+    backend.set_sims(scene)
+    backend.set_obstacles(scene)
+    backend.set_particles(scene)
+    backend.set_state(scene.pos, scene.quat, scene.vel, scene.omega)
+    return backend
+
+
+# ---------------------------------------------------------------------------
+# rotations
+def quat_to_matrix(q: np.ndarray) -> np.ndarray:
+    """Row-major body->world rotation matrices of unit quaternions (x,y,z,w)."""
+    x, y, z, w = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    m = np.empty(q.shape[:-1] + (9,))
+    m[..., 0] = 1 - 2 * (y * y + z * z)
+    m[..., 1] = 2 * (x * y - w * z)
+    m[..., 2] = 2 * (x * z + w * y)
+    m[..., 3] = 2 * (x * y + w * z)
+    m[..., 4] = 1 - 2 * (x * x + z * z)
+    m[..., 5] = 2 * (y * z - w * x)
+    m[..., 6] = 2 * (x * z - w * y)
+    m[..., 7] = 2 * (y * z + w * x)
+    m[..., 8] = 1 - 2 * (x * x + y * y)
+    return m
+
+
+def random_quaternions(rng, n):
+    """Uniform random unit quaternions (normalised 4-D Gaussians)."""
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q
+
+
+def transform12(pos, quat):
+    """Transform3 buffers (9 basis + 3 origin) of poses."""
+    pos = np.atleast_2d(pos)
+    quat = np.atleast_2d(quat)
+    t = np.empty((pos.shape[0], 12))
+    t[:, :9] = quat_to_matrix(quat)
+    t[:, 9:] = pos
+    return t
+
+
+def box_walls(lo, hi, material=0, density=0.0):
+    """Six Rectangle obstacles on the faces of the box [lo, hi]^3.
+
+    Returns (shapes, transforms): rectangle normal = third basis column
+    (CollisionDetection.cpp:67-69)."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    c = 0.5 * (lo + hi)
+    half = 0.5 * (hi - lo)
+    shapes, trs = [], []
+    # (axis of normal, coordinate) ; local x,y span the other two axes
+    rot = {
+        2: np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0]),   # normal z
+        0: np.array([0, 0, 1, 0, 1, 0, -1, 0, 0.0]),  # local z -> world x
+        1: np.array([1, 0, 0, 0, 0, 1, 0, -1, 0.0]),  # local z -> world y
+    }
+    for ax in (2, 0, 1):
+        others = [k for k in range(3) if k != ax]
+        R = rot[ax].reshape(3, 3)
+        # half lengths along local x,y = world extents of the mapped axes
+        lx = half[np.argmax(np.abs(R[:, 0]))]
+        ly = half[np.argmax(np.abs(R[:, 1]))]
+        for side in (lo[ax], hi[ax]):
+            o = c.copy()
+            o[ax] = side
+            shapes.append(dict(kind=A.RECTANGLE, material=material, dims=(lx, ly),
+                               crust=0.0, density=density))
+            trs.append(np.concatenate([rot[ax], o]))
+        del others
+    return shapes, np.array(trs)
+
+
+def hooke_row(kn, en, etat, muc, kr=0.0):
+    return [kn, en, etat, muc, kr]
+
+
+def matched_etat(kn, en, m_eff):
+    """Tangential damping rate equal to the normal one of the spring-dashpot:
+    etat = omega0 * |ln e| / sqrt(pi^2 + ln^2 e), omega0 = sqrt(kn/m*)
+    (HookeContactForceModel.cpp:56,122-134)."""
+    le = math.log(en)
+    return math.sqrt(kn / m_eff) * abs(le) / math.sqrt(math.pi ** 2 + le * le)
+
+
+# ---------------------------------------------------------------------------
+def config0(seed=0, n_side=10, walls=True, etat=None):
+    """1,000 monodisperse spheres (r = 0.5 mm, rho = 2500) on a 10^3 lattice,
+    spacing 1.2 mm, jitter U(-0.1, 0.1) mm, in a 12 mm walled box; kn = 1e5,
+    en = 0.8, mu = 0.3, dt = 1e-6, gravity -9.81 z."""
+    rng = np.random.default_rng(seed)
+    r, rho, sp = 0.5e-3, 2500.0, 1.2e-3
+    L = n_side * sp
+    g = np.arange(n_side)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    centres = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)[:, [2, 1, 0]]
+    pos = (centres + 0.5) * sp + rng.uniform(-0.1e-3, 0.1e-3, centres.shape)
+    n = pos.shape[0]
+    m = rho * 4.0 / 3.0 * math.pi * r ** 3
+    if etat is None:
+        etat = matched_etat(1e5, 0.8, 0.5 * m)
+    shapes = [dict(kind=A.SPHERE, material=0, dims=(r,), crust=0.05 * r, density=rho)]
+    obs_shape = np.zeros(0, np.uint32)
+    obs_tr = np.zeros((0, 12))
+    if walls:
+        wshapes, obs_tr = box_walls([0, 0, 0], [L, L, L], material=0, density=0.0)
+        obs_shape = np.arange(1, 1 + len(wshapes), dtype=np.uint32)
+        shapes += wshapes
+    q = np.zeros((n, 4))
+    q[:, 3] = 1.0
+    return Scene(
+        shapes=shapes, tables=np.array([hooke_row(1e5, 0.8, etat, 0.3, 0.0)]), n_materials=1,
+        sims=[dict(domain_min=(0.0, 0.0, 0.0), domain_max=(L, L, L), cell_size=0.0,
+                   cell_size_factor=1.0, gravity=(0.0, 0.0, -9.81), periodic=0)],
+        particle_shape=np.zeros(n, np.uint32), pos=pos, quat=q, vel=np.zeros((n, 3)),
+        omega=np.zeros((n, 3)), obstacle_shape=obs_shape, obstacle_tr=obs_tr, dt=1e-6,
+        name="c0_spheres_1000", notes=dict(etat=etat, steps=1000))
+
+
+def superquadric_params(rng, n, a_rng=(0.4e-3, 1.0e-3), n_rng=(2.0, 8.0)):
+    a = rng.uniform(*a_rng, size=(n, 3))
+    e = rng.uniform(*n_rng, size=(n, 2))
+    return np.concatenate([a, e], 1)
+
+
+def superquadric_scene(n, seed=1, spacing_factor=1.0, periodic=0b011, walls_z=True,
+                       vel_sigma=0.0, omega_sigma=0.0, crust_frac=0.05, rho=2500.0,
+                       kn=1e5, en=0.8, muc=0.3, etat=None, lattice=None, jitter=0.0,
+                       dt=1e-6, per_particle_shapes=True, n1n2=None, a_rng=(0.4e-3, 1.0e-3)):
+    """Superquadrics (a,b,c ~ U[0.4,1] mm, n1,n2 ~ U[2,8]) with uniform random
+    orientations on a cubic lattice.  `spacing_factor` scales the lattice
+    spacing relative to the largest circumscribed diameter (1.0: bounding
+    spheres cannot overlap).  Periodic in x,y (bitmask), floor/ceiling walls
+    in z when walls_z."""
+    from .shapes import superquadric_rc
+    rng = np.random.default_rng(seed)
+    prm = superquadric_params(rng, n, a_rng=a_rng)
+    if n1n2 is not None:
+        prm[:, 3:] = n1n2
+    rc = np.array([superquadric_rc(*p) for p in prm]) if n <= 20000 else superquadric_rc_vec(prm)
+    sp = spacing_factor * 2.0 * rc.max()
+    if lattice is None:
+        side = int(math.ceil(n ** (1.0 / 3.0)))
+        lattice = (side, side, int(math.ceil(n / (side * side))))
+    nx, ny, nz = lattice
+    idx = np.arange(nx * ny * nz)[:n]
+    cz, rem = np.divmod(idx, nx * ny)
+    cy, cx = np.divmod(rem, nx)
+    L = np.array([nx * sp, ny * sp, nz * sp])
+    pos = (np.stack([cx, cy, cz], 1) + 0.5) * sp
+    if jitter:
+        pos += rng.uniform(-jitter, jitter, pos.shape) * sp
+    quat = random_quaternions(rng, n)
+    crust = crust_frac * prm[:, :3].min(1)
+    if per_particle_shapes:
+        shapes = [dict(kind=A.SUPERQUADRIC, material=0, dims=tuple(p), crust=float(c), density=rho)
+                  for p, c in zip(prm, crust)]
+        pshape = np.arange(n, dtype=np.uint32)
+    else:
+        shapes = [dict(kind=A.SUPERQUADRIC, material=0, dims=tuple(prm[0]), crust=float(crust[0]),
+                       density=rho)]
+        pshape = np.zeros(n, np.uint32)
+    if etat is None:
+        from .shapes import superquadric_volume
+        m = rho * superquadric_volume(*prm[0])
+        etat = matched_etat(kn, en, 0.5 * m)
+    obs_shape = np.zeros(0, np.uint32)
+    obs_tr = np.zeros((0, 12))
+    if walls_z:
+        c = 0.5 * L
+        ws = [dict(kind=A.RECTANGLE, material=0, dims=(0.5 * L[0], 0.5 * L[1]), crust=0.0,
+                   density=0.0)] * 2
+        first = len(shapes)
+        shapes += ws
+        obs_shape = np.array([first, first + 1], np.uint32)
+        I = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0])
+        obs_tr = np.array([np.concatenate([I, [c[0], c[1], 0.0]]),
+                           np.concatenate([I, [c[0], c[1], L[2]]])])
+    vel = rng.standard_normal((n, 3)) * vel_sigma
+    omega = rng.standard_normal((n, 3)) * omega_sigma
+    sims = [dict(domain_min=(0.0, 0.0, 0.0), domain_max=tuple(L), cell_size=0.0,
+                 cell_size_factor=1.0, gravity=(0.0, 0.0, -9.81), periodic=periodic)]
+    return Scene(shapes=shapes, tables=np.array([hooke_row(kn, en, etat, muc, 0.0)]),
+                 n_materials=1, sims=sims, particle_shape=pshape, pos=pos, quat=quat, vel=vel,
+                 omega=omega, obstacle_shape=obs_shape, obstacle_tr=obs_tr, dt=dt,
+                 name=f"superquadrics_{n}", notes=dict(etat=etat, spacing=sp))
+
+
+def superquadric_rc_vec(prm):
+    from .shapes import superquadric_rc
+    return np.array([superquadric_rc(*p) for p in prm])

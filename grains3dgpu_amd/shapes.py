@@ -1,0 +1,70 @@
+"""Host-side shape helpers used when building scenes (circumscribed radius and
+volume of superquadrics, random convex hulls).  The kernels compute their own
+mass properties on the host side of the C-ABI; these helpers only size
+lattices and pick damping rates."""
+import math
+
+import numpy as np
+
+
+def _beta(x, y):
+    return math.exp(math.lgamma(x) + math.lgamma(y) - math.lgamma(x + y))
+
+
+def superquadric_volume(a, b, c, n1, n2):
+    """Superquadric<T>::computeVolume (Grains/Geometry/src/Superquadric.cpp:81-90)."""
+    e1, e2 = 2.0 / n1, 2.0 / n2
+    return 2.0 / 3.0 * a * b * c * e1 * e2 * _beta(e1, 0.5 * e1) * _beta(0.5 * e2, 0.5 * e2)
+
+
+def superquadric_rc(a, b, c, n1, n2):
+    """Superquadric<T>::computeCircumscribedRadius (Superquadric.cpp:122-155)."""
+    if n1 == 2.0 and n2 == 2.0:
+        return max(a, b, c)
+    if n1 == 2.0:
+        alpha = (b / a) ** (2.0 / (n2 - 2.0))
+        xt = 1.0 / (1.0 + alpha ** n2) ** (1.0 / n2)
+        return max(c, math.sqrt(a * xt * a * xt + alpha * b * xt * alpha * b * xt))
+    if n2 == 2.0:
+        m = max(a, b)
+        beta = (c * c / (m * m)) ** (1.0 / (n1 - 2.0))
+        xt = 1.0 / (1.0 + beta ** n1) ** (1.0 / n1)
+        return math.sqrt(m * xt * m * xt + beta * c * xt * beta * c * xt)
+    alpha = (b / a) ** (2.0 / (n2 - 2.0))
+    gamma = (1.0 + alpha ** n2) ** (n1 / n2 - 1.0)
+    beta = (gamma * c * c / (a * a)) ** (1.0 / (n1 - 1.0))
+    xt = 1.0 / ((1.0 + alpha ** n2) ** (n1 / n2) + beta ** n1) ** (1.0 / n1)
+    return math.sqrt(a * xt * a * xt + alpha * b * xt * alpha * b * xt + beta * c * xt * beta * c * xt)
+
+
+def random_hull_points(rng, n_points=8, radius=1.0):
+    """Points sampled uniformly on a sphere, shifted so that the volume
+    centroid of their convex hull is the origin (bodies rotate about it)."""
+    p = rng.standard_normal((n_points, 3))
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    p *= radius
+    c = hull_centroid(p)
+    return p - c
+
+
+def hull_centroid(p):
+    n = len(p)
+    V = 0.0
+    c = np.zeros(3)
+    for i in range(n):
+        for j in range(i + 1, n):
+            for k in range(j + 1, n):
+                a, b, cc = p[i], p[j], p[k]
+                nrm = np.cross(b - a, cc - a)
+                s = (p - a) @ nrm
+                s[[i, j, k]] = 0
+                tol = 1e-14 * np.linalg.norm(nrm)
+                pos, neg = (s > tol).any(), (s < -tol).any()
+                if pos and neg:
+                    continue
+                if pos:
+                    b, cc = cc, b
+                d = a @ np.cross(b, cc) / 6.0
+                V += d
+                c += d * (a + b + cc) / 4.0
+    return c / V

@@ -1,0 +1,243 @@
+/*
+ * grains_hip.h -- C-ABI of the MI355X DEM time-step library (libgrains_hip.so).
+ *
+ * One explicit DEM time step = linked-cell broad phase (cell hash, LDS radix
+ * sort, cell ranges) -> candidate pairs -> narrow phase (sphere analytic,
+ * rectangle plane, bounding sphere + OBB prefilter, GJK on support maps) ->
+ * Hooke spring-dashpot / Coulomb contact force -> gravity -> first-order
+ * explicit integration with quaternion rotation update.
+ *
+ * Every symbol here replaces a piece of the reference's C++ component manager
+ * (Grains/Component/include/ComponentManager.hh:24-184) and the objects it is
+ * handed (RigidBody, LinkedCell, ContactForceModel, TimeIntegrator).  The
+ * reference passes `T const* const*` arrays of polymorphic objects; this ABI
+ * passes plain arrays and sizes.  All functions are synchronous at the
+ * boundary (they return after the device work they enqueue has finished),
+ * return 0 on success and a negative GRAINS_E* code on failure, and never
+ * call exit() (the reference exits on configuration errors, e.g.
+ * Grains/Base/src/Grains.cpp:103,131).  The context owns all device memory;
+ * caller buffers are host memory unless the name says `_device`.
+ *
+ * Conventions
+ *  - lengths in the caller's units (the reference uses SI), FP64 at the ABI;
+ *  - quaternions are (x, y, z, w): w last, as Quaternion<T>::operator[]
+ *    (Grains/Geometry/src/Quaternion.cpp:316-319);
+ *  - a "transform" is the 12-double Transform3 buffer of the reference
+ *    (Grains/Geometry/src/Transform3.cpp:26-29,133-137): row-major 3x3 basis
+ *    (body -> world) followed by the origin;
+ *  - pairs (i, j) use particle indices as given to grains_set_particles.
+ */
+#ifndef GRAINS_HIP_H
+#define GRAINS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define GRAINS_OK 0
+#define GRAINS_EINVAL (-1)     /* bad argument or configuration            */
+#define GRAINS_ENOMEM (-2)     /* device or host allocation failed         */
+#define GRAINS_EDEVICE (-3)    /* HIP runtime error                        */
+#define GRAINS_ESTATE (-4)     /* call out of order (e.g. step before set) */
+#define GRAINS_EOVERFLOW (-5)  /* a fixed capacity was exceeded            */
+#define GRAINS_EUNSUPPORTED (-6)
+
+/* ---- convex kinds: numbering of ConvexType, Grains/Geometry/include/Convex.hh:7-16 */
+#define GRAINS_SPHERE 0
+#define GRAINS_BOX 1
+#define GRAINS_CYLINDER 2      /* not on this path: rejected with EUNSUPPORTED */
+#define GRAINS_CONE 3          /* not on this path: rejected with EUNSUPPORTED */
+#define GRAINS_SUPERQUADRIC 4
+#define GRAINS_POLYHEDRON 5    /* enum value exists in the reference, class does not */
+#define GRAINS_RECTANGLE 6
+
+/* ---- arithmetic of the narrow phase and force ---------------------------- */
+#define GRAINS_FP64 0          /* plain: everything in FP64                  */
+#define GRAINS_FP32_MIXED 1    /* GJK, support maps, forces in FP32; FP64
+                                  state, accumulators, cell keys, and an FP64
+                                  re-run of GJK near degenerate simplices    */
+
+/* ---- semantics flags (grains_config_t.flags) ---------------------------- */
+/* Reproduce the reference narrow phase exactly: GJK_JH with its early exit
+ * on mu < 1e-4*LOWEPS (GJK_JH.cpp:370-372), no crust core
+ * (CollisionDetection.cpp:231-236 commented out), and the relative velocity
+ * taken with the world contact point as lever arm
+ * (ComponentManagerCPU.cpp:404-407, Kinematics.cpp:86-91).  Without it the
+ * "physics" semantics apply: GJK on the crust cores with the standard
+ * van den Bergen termination and the lever arm cp - x_i. */
+#define GRAINS_REF_QUIRKS 0x1u
+/* Nesterov-accelerated support directions in GJK (physics semantics only). */
+#define GRAINS_NESTEROV 0x2u
+
+/* ---- pair lists ----------------------------------------------------------- */
+#define GRAINS_PAIRS_CANDIDATE 0 /* passed the bounding-sphere prefilter     */
+#define GRAINS_PAIRS_CONTACT 1   /* overlap distance < 0                     */
+
+/* One particle or obstacle shape, the data of RigidBody<T,T>
+ * (Grains/Geometry/src/RigidBody.cpp:19-59) plus its Convex. */
+typedef struct grains_shape_t {
+  int32_t kind;          /* GRAINS_SPHERE ... GRAINS_RECTANGLE                */
+  int32_t material;      /* material id; contact table index via computeHash  */
+  double dims[5];        /* SPHERE r | BOX ex,ey,ez (half) | SUPERQUADRIC
+                            a,b,c,n1,n2 | RECTANGLE LX,LY (half)             */
+  double crust;          /* CrustThickness                                    */
+  double density;        /* 0 gives mass 0 (obstacles)                        */
+  int32_t n_vertices;    /* POLYHEDRON: vertex count (convex hull points)     */
+  int32_t vertex_offset; /* POLYHEDRON: first (x,y,z) triplet in the array
+                            passed to grains_set_shapes                      */
+} grains_shape_t;
+
+/* Hooke parameters of one material pair, HookeContactForceModel ctor
+ * (Grains/Component/src/HookeContactForceModel.cpp:48-57). */
+typedef struct grains_hooke_t {
+  double kn, en, etat, muc, kr;
+} grains_hooke_t;
+
+/* One independent simulation (a LinkedCell domain plus its forces).
+ * Several of them run batched in one context. */
+typedef struct grains_sim_t {
+  double domain_min[3];     /* Origin OX,OY,OZ (Grains.cpp:114-120)         */
+  double domain_max[3];     /* LinkedCell MX,MY,MZ (Grains.cpp:108-112)      */
+  double cell_size;         /* > 0: explicit cell extent; 0: derived as
+                               max(1,CellSizeFactor) * 2 * max R_c
+                               (Grains.cpp:226-245)                          */
+  double cell_size_factor;  /* CellSizeFactor                                */
+  double gravity[3];        /* Forces/Gravity (Grains.cpp:305-336)           */
+  int32_t periodic;         /* bitmask 1 x, 2 y, 4 z: periodic directions
+                               (not in the reference: Grains.cpp:129-133
+                               aborts); each needs >= 3 cells               */
+  int32_t contact_table;    /* which table block of grains_set_contact_tables */
+} grains_sim_t;
+
+typedef struct grains_config_t {
+  double dt;                /* TimeInterval dt                               */
+  int32_t precision;        /* GRAINS_FP64 | GRAINS_FP32_MIXED               */
+  uint32_t flags;           /* GRAINS_REF_QUIRKS | GRAINS_NESTEROV           */
+  double gjk_rel_tol;       /* 0: LOWEPS of the precision (Basic.hh:126-128) */
+  int32_t max_neighbors;    /* candidate capacity per particle, 0: 48        */
+  int32_t block_size;       /* threads per block of the per-particle kernels;
+                               0: 256                                        */
+  int32_t reserved[4];
+} grains_config_t;
+
+/* Counters of the last grains_step call (summed over its steps). */
+typedef struct grains_stats_t {
+  uint64_t steps;
+  uint64_t candidate_pairs;   /* ordered pairs passing the prefilter          */
+  uint64_t gjk_calls;         /* ordered pairs that ran GJK                   */
+  uint64_t gjk_iterations;    /* summed GJK iterations                        */
+  uint64_t contacts;          /* ordered particle-particle contacts           */
+  uint64_t wall_contacts;     /* particle-obstacle contacts                   */
+  uint64_t out_of_domain;     /* particle-steps whose centre left the grid    */
+  uint64_t deep_contacts;     /* physics semantics: crust cores overlapped    */
+  uint64_t fp64_refines;      /* FP32-mixed: GJK re-run in FP64               */
+  uint64_t neighbor_overflow; /* candidates dropped: capacity exceeded         */
+  uint32_t max_candidates;    /* largest per-particle candidate count seen    */
+  uint32_t sort_passes;       /* radix passes per step                         */
+  double   elapsed_ms;        /* device time of the last grains_step call      */
+} grains_stats_t;
+
+typedef struct grains_ctx grains_ctx_t;
+
+/* Context ---------------------------------------------------------------- */
+/* Replaces GrainsGPU::initialize/setupGPUDevice (Grains/Base/src/GrainsGPU.cpp:27-85). */
+int grains_create(const grains_config_t* cfg, grains_ctx_t** out);
+void grains_destroy(grains_ctx_t* ctx);
+const char* grains_last_error(const grains_ctx_t* ctx);
+/* Library version string, e.g. "grains_hip 0.1 gfx950". */
+const char* grains_version(void);
+
+/* Scene ------------------------------------------------------------------ */
+/* Shapes: RigidBody table (Grains.cpp:164-218, RigidBodyGPUWrapper.cpp:20-181).
+ * `vertices` holds (x,y,z) triplets for POLYHEDRON shapes, may be NULL. */
+int grains_set_shapes(grains_ctx_t* ctx, const grains_shape_t* shapes,
+                      uint32_t n_shapes, const double* vertices,
+                      uint32_t n_vertices);
+/* Contact table blocks, each n_materials*(n_materials+1)/2 entries indexed by
+ * ContactForceModelBuilderFactory::computeHash(matA, matB) = l(l+1)/2 + s
+ * (Grains/Component/src/ContactForceModelBuilderFactory.cpp:77-83). */
+int grains_set_contact_tables(grains_ctx_t* ctx, const grains_hooke_t* tables,
+                              uint32_t n_materials, uint32_t n_tables);
+/* Simulations (LinkedCell<T> ctor, Grains/Component/src/LinkedCell.cpp:16-73). */
+int grains_set_sims(grains_ctx_t* ctx, const grains_sim_t* sims, uint32_t n_sims);
+/* Obstacles: ComponentManager::initializeObstacles (ComponentManagerCPU.cpp:215-231).
+ * `transforms` is n x 12 (Transform3 buffer). `sim` may be NULL (all sim 0). */
+int grains_set_obstacles(grains_ctx_t* ctx, const uint32_t* shape,
+                         const uint32_t* sim, const double* transforms,
+                         uint32_t n_obstacles);
+/* Particles: ComponentManager::initializeParticles (ComponentManagerCPU.cpp:236-252).
+ * Particles of one sim must be contiguous and sims in ascending order.
+ * `sim` may be NULL (all sim 0).  Resets the state to rest at the origin. */
+int grains_set_particles(grains_ctx_t* ctx, const uint32_t* shape,
+                         const uint32_t* sim, uint64_t n);
+
+/* State ------------------------------------------------------------------ */
+/* pos n x 3, quat n x 4 (x,y,z,w), vel n x 3, omega n x 3; NULL = unchanged. */
+int grains_set_state(grains_ctx_t* ctx, const double* pos, const double* quat,
+                     const double* vel, const double* omega, uint64_t n);
+int grains_get_state(grains_ctx_t* ctx, double* pos, double* quat, double* vel,
+                     double* omega, uint64_t n);
+/* ComponentManager::setTransform/getTransform (ComponentManager.hh:46,85):
+ * n x 12 Transform3 buffers. */
+int grains_set_transforms(grains_ctx_t* ctx, const double* transforms, uint64_t n);
+int grains_get_transforms(grains_ctx_t* ctx, double* transforms, uint64_t n);
+
+/* Time stepping ------------------------------------------------------------ */
+/* n_steps iterations of the GrainsCPU::simulate loop body
+ * (Grains/Base/src/GrainsCPU.cpp:55-66): detectCollisionAndComputeContactForces,
+ * addExternalForces, moveParticles. */
+int grains_step(grains_ctx_t* ctx, uint32_t n_steps);
+/* Same, replayed from a captured HIP graph of one step. */
+int grains_step_graph(grains_ctx_t* ctx, uint32_t n_steps);
+
+/* Results of the most recent step's broad and narrow phase ------------------ */
+/* keys[i]: linear cell hash of particle i (LinkedCell::computeLinearCellHash,
+ * LinkedCell.cpp:117-149; offset by the sim's first cell when batched).
+ * sorted[k]: stable (key, index) order.  Either pointer may be NULL. */
+int grains_get_cell_keys(grains_ctx_t* ctx, uint32_t* keys, uint32_t* sorted, uint64_t n);
+/* Pairs (i, j) as 2*m uint32, ordered by i then by the reference visiting
+ * order (27 cells k,j,i outer->inner, residents ascending; ComponentManagerCPU.cpp:364-421).
+ * *m receives the total count; at most `cap` pairs are written. */
+int grains_get_pairs(grains_ctx_t* ctx, int which, uint32_t* ij, uint64_t cap, uint64_t* m);
+/* Cumulative per-particle contact counts, the `result` array of
+ * detectCollisionAndComputeContactForcesParticles (ComponentManagerCPU.cpp:419). */
+int grains_get_contact_count(grains_ctx_t* ctx, int32_t* counts, uint64_t n);
+int grains_reset_contact_count(grains_ctx_t* ctx);
+/* Torce (force, torque) accumulated by the last step before the integrator
+ * reset it (ComponentManagerCPU.cpp:428-460, 483). n x 3 each. */
+int grains_get_torce(grains_ctx_t* ctx, double* force, double* torque, uint64_t n);
+int grains_get_stats(grains_ctx_t* ctx, grains_stats_t* stats);
+
+/* Narrow phase alone ------------------------------------------------------- */
+/* GJK distance between shape_a[k] at transform ta[k] and shape_b[k] at tb[k]
+ * (12-double Transform3 buffers), the GrainsTestDev microbenchmark
+ * (Grains/Base/src/GrainsTestDev.cpp:35-55, 168-185) calling
+ * computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391).  Returns distance, the
+ * witness points in world coordinates (may be NULL) and iteration counts
+ * (may be NULL).  Honours precision, REF_QUIRKS and NESTEROV of the context;
+ * shapes are the full shapes (no crust core). */
+int grains_gjk_batch(grains_ctx_t* ctx, const uint32_t* shape_a,
+                     const uint32_t* shape_b, const double* ta,
+                     const double* tb, uint64_t n, double* dist, double* pa,
+                     double* pb, int32_t* iters);
+/* Device-resident variant for timing: all pointers are device pointers. */
+int grains_gjk_batch_device(grains_ctx_t* ctx, const uint32_t* shape_a_device,
+                            const uint32_t* shape_b_device, const double* ta_device,
+                            const double* tb_device, uint64_t n,
+                            double* dist_device, int32_t* iters_device,
+                            float* elapsed_ms);
+
+/* Checkpoint / restore of the particle state (pos, quat, vel, omega, contact
+ * counts).  `bytes` = NULL with *size queries the size. */
+int grains_snapshot(grains_ctx_t* ctx, void* bytes, size_t* size);
+int grains_restore(grains_ctx_t* ctx, const void* bytes, size_t size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAINS_HIP_H */

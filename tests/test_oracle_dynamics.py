@@ -1,0 +1,209 @@
+"""Pins of the oracle's force and integration against closed forms:
+Hooke spring-dashpot restitution and Coulomb cap
+(HookeContactForceModel.cpp:91-186), explicit Euler free fall
+(FirstOrderExplicit.cpp:46-60), free rotation through the quaternion
+increment (TimeIntegrator.cpp:28-44, Transform3.cpp:229-246),
+Quaternion(Matrix3) branches (Quaternion.cpp:125-169) and mass properties
+(Sphere.cpp:72-89, Box.cpp:81-107, Superquadric.cpp:81-155, RigidBody.cpp:19-59)."""
+import math
+
+import numpy as np
+import pytest
+
+from grains3dgpu_amd import _abi as A
+from grains3dgpu_amd import scene as S
+from tests.test_oracle_narrowphase import sq_implicit, sq_surface_samples
+
+
+def two_body(pos, vel, omega=None, r=0.5e-3, rho=2500.0, kn=1e5, en=0.8, etat=0.0, muc=0.3,
+             g=(0, 0, 0), dt=1e-9, obstacles=None, flags=0):
+    from oracle import pyoracle as P
+    n = len(pos)
+    q = np.tile([0, 0, 0, 1.0], (n, 1))
+    shapes = [dict(kind=A.SPHERE, dims=(r,), density=rho)]
+    osh, otr = np.zeros(0, np.uint32), np.zeros((0, 12))
+    if obstacles is not None:
+        shapes += obstacles[0]
+        osh = np.arange(1, 1 + len(obstacles[0]), dtype=np.uint32)
+        otr = obstacles[1]
+    sc = S.Scene(shapes=shapes, tables=np.array([S.hooke_row(kn, en, etat, muc)]), n_materials=1,
+                 sims=[dict(domain_min=(-5e-3,) * 3, domain_max=(5e-3,) * 3, gravity=g)],
+                 particle_shape=np.zeros(n, np.uint32), pos=np.array(pos, float), quat=q,
+                 vel=np.array(vel, float),
+                 omega=np.zeros((n, 3)) if omega is None else np.array(omega, float),
+                 obstacle_shape=osh, obstacle_tr=otr, dt=dt)
+    o = P.Oracle(dt=dt, flags=flags)
+    S.load(o, sc)
+    return o
+
+
+@pytest.mark.parametrize("en", [0.8, 0.5])
+def test_binary_collision_restitution(oracle_lib, en):
+    """Linear spring-dashpot with the en -> damping map of the reference gives
+    exactly restitution en for the continuous contact; explicit Euler at
+    omega0*dt ~ 4e-4 stays within 0.3 %."""
+    v0 = 0.1
+    o = two_body([[-0.5e-3 - 1e-7, 0, 0], [0.5e-3 + 1e-7, 0, 0]], [[v0, 0, 0], [-v0, 0, 0]], en=en)
+    for _ in range(40):
+        o.step(500)
+        _, _, v, _ = o.get_state()
+        p, _, _, _ = o.get_state()
+        if p[1, 0] - p[0, 0] > 1e-3 + 1e-8 and v[0, 0] < 0:
+            break
+    _, _, v, _ = o.get_state()
+    e_meas = (v[1, 0] - v[0, 0]) / (2 * v0)
+    assert abs(e_meas - en) < 3e-3 * en
+    # momentum is exchanged symmetrically
+    assert abs(v[0, 0] + v[1, 0]) < 1e-12
+
+
+def test_coulomb_cap_and_normal_force(oracle_lib):
+    from oracle import pyoracle as P
+    prm = (1e5, 0.8, 1e4, 0.3, 0.0)
+    cv = np.array([0, 0, 2e-6])
+    cp = np.array([0.1, 0.2, 0.3])
+    rv = np.array([5.0, 0.0, -0.01])   # fast sliding in x, approaching in z
+    m1, m2 = 2e-6, 3e-6
+    f, t = P.hooke(prm, cp, cv, -2e-6, rv, [0, 0, 0], m1, m2, [0.1, 0.2, 0.25])
+    mstar = m1 * m2 / (m1 + m2)
+    w0 = math.sqrt(1e5 / mstar)
+    zeta = -math.log(0.8) / math.sqrt(math.pi ** 2 + math.log(0.8) ** 2)
+    fn = 1e5 * 2e-6 + 2 * zeta * w0 * mstar * 0.01
+    np.testing.assert_allclose(f[2], fn, rtol=1e-12)
+    np.testing.assert_allclose(f[0], -0.3 * fn, rtol=1e-12)   # saturated: -mu |FN| t
+    np.testing.assert_allclose(t, np.cross(cp - [0.1, 0.2, 0.25], f), rtol=1e-12)
+    # slow sliding: viscous tangential force -2 etat m* v_t
+    f, _ = P.hooke(prm, cp, cv, -2e-6, [1e-4, 0, 0], [0, 0, 0], m1, m2, [0, 0, 0])
+    np.testing.assert_allclose(f[0], -2 * 1e4 * mstar * 1e-4, rtol=1e-12)
+    # obstacle of mass 0: m* = m1/2, omega0 = 2 sqrt(kn/m*) (lines 122-128)
+    f, _ = P.hooke(prm, cp, cv, -2e-6, [0, 0, -0.01], [0, 0, 0], m1, 0.0, [0, 0, 0])
+    ms = 0.5 * m1
+    np.testing.assert_allclose(f[2], 1e5 * 2e-6 + 2 * zeta * 2 * math.sqrt(1e5 / ms) * ms * 0.01,
+                               rtol=1e-12)
+
+
+def test_free_fall_explicit_euler_closed_form(oracle_lib):
+    g = -9.81
+    dt = 1e-5
+    o = two_body([[0, 0, 1e-3]], [[0.02, 0, 0.3]], g=(0, 0, g), dt=dt)
+    n = 250
+    o.step(n)
+    p, _, v, _ = o.get_state()
+    # x_n = x0 + n dt v0 + g dt^2 n(n-1)/2 ; v_n = v0 + n g dt
+    np.testing.assert_allclose(p[0], [0.02 * n * dt, 0, 1e-3 + 0.3 * n * dt + g * dt * dt * n * (n - 1) / 2],
+                               rtol=1e-11, atol=1e-17)
+    np.testing.assert_allclose(v[0], [0.02, 0, 0.3 + n * g * dt], rtol=1e-12)
+
+
+def test_free_rotation_closed_form(oracle_lib):
+    w0 = np.array([30.0, -40.0, 120.0])
+    dt = 1e-4
+    o = two_body([[0, 0, 0]], [[0, 0, 0]], omega=[w0], dt=dt)
+    n = 500
+    o.step(n)
+    tr = o.get_transforms()[0]
+    ang = np.linalg.norm(w0) * n * dt
+    k = w0 / np.linalg.norm(w0)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    R = np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+    np.testing.assert_allclose(tr[:9].reshape(3, 3), R, atol=1e-12)
+    _, _, _, om = o.get_state()
+    np.testing.assert_allclose(om[0], w0, rtol=1e-13)   # isotropic inertia: no gyroscopic term
+
+
+def test_sphere_rests_on_floor(oracle_lib):
+    """Damped settling on a mass-0 floor ends at overlap m g / kn."""
+    r, rho, kn = 0.5e-3, 2500.0, 10.0
+    m = rho * 4 / 3 * math.pi * r ** 3
+    floor = ([dict(kind=A.RECTANGLE, dims=(5e-3, 5e-3), density=0.0)],
+             np.array([[1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, -1e-3]], float))
+    o = two_body([[0, 0, -1e-3 + r]], [[0, 0, 0]], kn=kn, en=0.3, g=(0, 0, -9.81), dt=2e-5,
+                 obstacles=floor)
+    o.step(20000)
+    p, _, v, _ = o.get_state()
+    np.testing.assert_allclose(-1e-3 + r - p[0, 2], m * 9.81 / kn, rtol=1e-6)
+    assert abs(v[0, 2]) < 1e-9
+
+
+@pytest.mark.parametrize("dom", [3, 0, 1, 2])
+def test_quaternion_from_matrix_branches(oracle_lib, dom):
+    from oracle import pyoracle as P
+    rng = np.random.default_rng(dom)
+    for _ in range(20):
+        q = rng.standard_normal(4) * 0.2
+        q[dom] = 1.0 + rng.uniform()          # make one component dominant
+        q /= np.linalg.norm(q)
+        m = P.matrix_from_quat(q)
+        np.testing.assert_allclose(m, S.quat_to_matrix(q), atol=1e-15)
+        r = P.quat_from_matrix(m)
+        s = 1.0 if r @ q > 0 else -1.0
+        np.testing.assert_allclose(r, s * q, atol=1e-14)
+
+
+def test_euler_angle_basis(oracle_lib):
+    """setBasis(aX,aY,aZ) = Rz(aZ) Ry(aY) Rx(aX) (Transform3.cpp:151-162)."""
+    from oracle import pyoracle as P
+    ax, ay, az = 0.3, -1.1, 2.0
+    Rx = np.array([[1, 0, 0], [0, math.cos(ax), -math.sin(ax)], [0, math.sin(ax), math.cos(ax)]])
+    Ry = np.array([[math.cos(ay), 0, math.sin(ay)], [0, 1, 0], [-math.sin(ay), 0, math.cos(ay)]])
+    Rz = np.array([[math.cos(az), -math.sin(az), 0], [math.sin(az), math.cos(az), 0], [0, 0, 1]])
+    np.testing.assert_allclose(P.basis_from_angles(ax, ay, az).reshape(3, 3), Rz @ Ry @ Rx, atol=1e-15)
+
+
+def test_mass_properties_closed_forms(oracle_lib):
+    from oracle import pyoracle as P
+    r, rho = 0.7, 3.0
+    p = P.shape_props(dict(kind=A.SPHERE, dims=(r,), density=rho))
+    m = rho * 4 / 3 * math.pi * r ** 3
+    np.testing.assert_allclose([p["volume"] * rho, p["mass"]], [m, m], rtol=1e-14)
+    np.testing.assert_allclose(p["inertia"][[0, 3, 5]], 0.4 * m * r * r, rtol=1e-14)
+    np.testing.assert_allclose(p["inertia_inv"][[0, 3, 5]], 1 / (0.4 * m * r * r), rtol=1e-14)
+    e = (0.3, 0.5, 0.2)
+    p = P.shape_props(dict(kind=A.BOX, dims=e, density=rho))
+    m = rho * 8 * np.prod(e)
+    np.testing.assert_allclose(p["inertia"][[0, 3, 5]],
+                               m / 3 * np.array([e[1] ** 2 + e[2] ** 2, e[0] ** 2 + e[2] ** 2,
+                                                 e[0] ** 2 + e[1] ** 2]), rtol=1e-14)
+    assert abs(p["rc"] - np.linalg.norm(e)) < 1e-15
+    # the same cube as a polyhedron of its 8 corners
+    cor = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]) * e
+    pp = P.shape_props(dict(kind=A.POLYHEDRON, dims=(0,) * 5, density=rho, n_vertices=8,
+                            vertex_offset=0), vertices=cor)
+    np.testing.assert_allclose(pp["volume"], p["volume"], rtol=1e-13)
+    np.testing.assert_allclose(pp["inertia"], p["inertia"], rtol=1e-12, atol=1e-14)
+    # ellipsoid as superquadric n1 = n2 = 2
+    a, b, c = 0.7, 0.4, 0.55
+    p = P.shape_props(dict(kind=A.SUPERQUADRIC, dims=(a, b, c, 2, 2), density=rho))
+    m = rho * 4 / 3 * math.pi * a * b * c
+    np.testing.assert_allclose(p["mass"], m, rtol=1e-13)
+    np.testing.assert_allclose(p["inertia"][[0, 3, 5]],
+                               m / 5 * np.array([b * b + c * c, a * a + c * c, a * a + b * b]), rtol=1e-13)
+    assert p["rc"] == max(a, b, c)
+
+
+@pytest.mark.parametrize("n1,n2", [(3.0, 3.0), (6.0, 2.5), (2.0, 5.0), (4.0, 2.0)])
+def test_superquadric_mass_by_quadrature(oracle_lib, n1, n2):
+    from oracle import pyoracle as P
+    a, b, c = 0.7, 0.4, 0.55
+    p = P.shape_props(dict(kind=A.SUPERQUADRIC, dims=(a, b, c, n1, n2), density=1.0))
+    m = 160
+    h = 2.0 / m
+    g = -1 + h * (np.arange(m) + 0.5)
+    X, Y, Z = np.meshgrid(g * a, g * b, g * c, indexing="ij")
+    pts = np.stack([X, Y, Z], -1)
+    inside = sq_implicit(pts, a, b, c, n1, n2) <= 1.0
+    dv = (h * a) * (h * b) * (h * c)
+    vol = inside.sum() * dv
+    Ixx = ((Y ** 2 + Z ** 2) * inside).sum() * dv
+    Izz = ((X ** 2 + Y ** 2) * inside).sum() * dv
+    np.testing.assert_allclose(p["volume"], vol, rtol=5e-3)
+    np.testing.assert_allclose(p["inertia"][0], Ixx, rtol=5e-3)
+    np.testing.assert_allclose(p["inertia"][5], Izz, rtol=5e-3)
+    # exact circumscribed radius = farthest surface point; the reference's
+    # formula (REF_QUIRKS) is never larger and is smaller when its beta
+    # exponent matters (n1 > 2, n2 > 2)
+    far = np.linalg.norm(sq_surface_samples(a, b, c, n1, n2, m=400), axis=1).max()
+    assert far <= p["rc_exact"] * (1 + 1e-9) and far >= p["rc_exact"] * (1 - 2e-3)
+    assert p["rc"] <= p["rc_exact"] * (1 + 1e-12)
+    if n1 > 2 and n2 > 2:
+        assert p["rc"] < far
