@@ -1,0 +1,357 @@
+// gr_broad.hip -- linked-cell broad phase on the device.
+//
+//   k_hash        cell key of every particle (LinkedCell::computeLinearCellHash,
+//                 LinkedCell.cpp:117-149), FP64 divide + floor as the reference
+//   radix sort    stable LSD sort of (key, index), 8-bit digits: per-tile LDS
+//                 histogram, device-wide exclusive scan, stable scatter ranked
+//                 with wave ballots (replaces thrust::sort_by_key,
+//                 ComponentManagerGPU.cpp:282-285)
+//   k_cells       cell start/end by adjacent difference
+//                 (ComponentManagerGPU_Kernels.cpp:28-75) and the inverse
+//                 permutation
+//   k_candidates  per particle, the 27 cells in the reference order
+//                 (LinkedCell.cpp:41-70) and their residents ascending
+//                 (ComponentManagerCPU.cpp:364-421), bounding-sphere prefilter
+//   scan/expand   CSR offsets and the flat (i, j) entry list
+#include "gr_internal.hpp"
+
+namespace gr {
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, p, n, v);
+}
+
+// ---- cell hash --------------------------------------------------------------------
+__global__ void k_hash(uint64_t n, const double* __restrict__ pos, const uint32_t* __restrict__ sim,
+                       const SimGrid* __restrict__ grids, uint32_t* __restrict__ key,
+                       unsigned long long* __restrict__ counters) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t ood = 0;
+  if (i < n) {
+    const SimGrid& g = grids[sim[i]];
+    const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+    uint32_t c[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double f = floor((p[k] - g.lo[k]) / g.h);
+      if (!(f >= 0.0)) {
+        c[k] = 0;
+        ood = 1;
+      } else if (f >= (double)g.n3[k]) {
+        c[k] = g.n3[k] - 1;
+        ood = 1;
+      } else {
+        c[k] = (uint32_t)f;
+      }
+    }
+    key[i] = g.base + (c[2] * g.n3[1] + c[1]) * g.n3[0] + c[0];
+  }
+  count_add(&counters[kCntOod], ood);
+}
+
+void launch_hash(const DevScene& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_hash, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d.n,
+                     d.pos, d.sim, d.grids, d.key, d.counters);
+}
+
+// ---- block-wide exclusive scan of one value per thread ---------------------------
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / 64;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds_waves[wave] = x;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  const int nw = blockDim.x / 64;
+  for (int w = 0; w < nw; ++w) {
+    uint32_t t = lds_waves[w];
+    if (w < wave) off += t;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return off + x - v;
+}
+
+// ---- radix sort -----------------------------------------------------------------
+// Elements of tile b are b*kTile + k*kBlock + tid, k = 0..kItems-1: the item
+// slot k outermost, so a stable rank follows (k, wave, lane).
+__global__ void __launch_bounds__(kBlock) k_radix_hist(uint64_t n, const uint32_t* __restrict__ keys,
+                                                      int shift, uint32_t ntiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[256];
+  const uint32_t tid = threadIdx.x;
+  cnt[tid] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    uint64_t e = base + (uint64_t)k * kBlock + tid;
+    if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)tid * ntiles + blockIdx.x] = cnt[tid];
+}
+
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(uint64_t n, const uint32_t* __restrict__ kin,
+                                                         const uint32_t* __restrict__ iin, int shift,
+                                                         uint32_t ntiles, const uint32_t* __restrict__ offs,
+                                                         uint32_t* __restrict__ kout, uint32_t* __restrict__ iout,
+                                                         int identity_in) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[kBlock / 64][256];
+  const uint32_t tid = threadIdx.x;
+  const int lane = __lane_id();
+  const int wave = tid / 64;
+  run[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) wcnt[w][tid] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int k = 0; k < kItems; ++k) {
+    const uint64_t e = base + (uint64_t)k * kBlock + tid;
+    const bool valid = e < n;
+    uint32_t key = valid ? kin[e] : 0u;
+    uint32_t idx = valid ? (identity_in ? (uint32_t)e : iin[e]) : 0u;
+    uint32_t dig = (key >> shift) & 255u;
+    // lanes of this wave holding the same digit
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = __ballot((dig >> b) & 1u);
+      m &= ((dig >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = __popcll(m & lt);
+    if (valid && rank == 0) wcnt[wave][dig] = __popcll(m);
+    __syncthreads();
+    // per digit: offsets of the waves in order, then advance the running count
+    {
+      uint32_t r = run[tid];
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; ++w) {
+        uint32_t c = wcnt[w][tid];
+        wcnt[w][tid] = r;
+        r += c;
+      }
+      run[tid] = r;
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t p = wcnt[wave][dig] + rank;
+      kout[p] = key;
+      iout[p] = idx;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+  }
+}
+
+// ---- device-wide exclusive scan (reduce, top, down-sweep) -------------------------
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(uint64_t n, const uint32_t* __restrict__ in,
+                                                       uint32_t* __restrict__ bsum) {
+  __shared__ uint32_t lw[kBlock / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k)
+    if (base + k < n) s += in[base + k];
+  uint32_t tot;
+  block_exclusive_scan(s, lw, tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// one block: exclusive scan of nb block sums in place, total in bsum[nb]
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t nb, uint32_t* __restrict__ bsum) {
+  __shared__ uint32_t lw[16];
+  uint32_t carry = 0;
+  for (uint32_t c = 0; c < nb; c += 1024) {
+    uint32_t i = c + threadIdx.x;
+    uint32_t v = i < nb ? bsum[i] : 0u;
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan(v, lw, tot);
+    if (i < nb) bsum[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_down(uint64_t n, const uint32_t* __restrict__ in,
+                                                     uint32_t* __restrict__ out,
+                                                     const uint32_t* __restrict__ bsum, uint32_t nb) {
+  __shared__ uint32_t lw[kBlock / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+  uint32_t v[kItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t ex = block_exclusive_scan(s, lw, tot) + bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+// out has n+1 entries; block_sums needs ceil(n/kTile)+1 entries
+void launch_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* block_sums, hipStream_t s) {
+  uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, n, in, block_sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, nb, block_sums);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, n, in, out, block_sums, nb);
+}
+
+void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx_sorted, hipStream_t s) {
+  const uint64_t n = d.n;
+  const uint32_t nt = (uint32_t)((n + kTile - 1) / kTile);
+  const uint32_t* kin = d.key;
+  const uint32_t* iin = nullptr;
+  uint32_t* ko[2] = {d.keyA, d.keyB};
+  uint32_t* io[2] = {d.idxA, d.idxB};
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(kBlock), 0, s, n, kin, shift, nt, d.hist);
+    launch_scan(d.hist, d.hist, (uint64_t)256 * nt, d.scan_blocks, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(kBlock), 0, s, n, kin, iin, shift, nt, d.hist,
+                       ko[p & 1], io[p & 1], p == 0 ? 1 : 0);
+    kin = ko[p & 1];
+    iin = io[p & 1];
+  }
+  *keys_sorted = const_cast<uint32_t*>(kin);
+  *idx_sorted = const_cast<uint32_t*>(iin);
+}
+
+// ---- cell ranges --------------------------------------------------------------------
+__global__ void k_cells(uint64_t n, const uint32_t* __restrict__ ks, const uint32_t* __restrict__ is,
+                        uint32_t* __restrict__ cs, uint32_t* __restrict__ ce, uint32_t* __restrict__ rank) {
+  uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint32_t k = ks[s];
+  if (s == 0 || ks[s - 1] != k) cs[k] = (uint32_t)s;
+  if (s == n - 1 || ks[s + 1] != k) ce[k] = (uint32_t)s + 1;
+  rank[is[s]] = (uint32_t)s;
+}
+
+void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
+  // empty cells: start = end = 0
+  launch_fill_u32(d.cell_start, d.ncells, 0u, s);
+  launch_fill_u32(d.cell_end, d.ncells, 0u, s);
+  hipLaunchKernelGGL(k_cells, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d.n,
+                     keys_s, idx_s, d.cell_start, d.cell_end, d.rank);
+}
+
+// ---- candidates -------------------------------------------------------------------
+// One thread per sorted position s (spatially coherent waves).  Writes the
+// candidates of particle idx_s[s] into cand[s*K ...] in the reference visit
+// order and their count into cand_count[s].
+__global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_t* __restrict__ ks,
+                                                      const uint32_t* __restrict__ is) {
+  const uint64_t n = d.n;
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cnt = 0, over = 0;
+  if (s < n) {
+    const uint32_t i = is[s];
+    const SimGrid g = d.grids[d.sim[i]];
+    const V3<double> xi = load3(d.pos, n, i);
+    const double ri = d.rc_pf[d.shape[i]];
+    const uint32_t local = ks[s] - g.base;
+    const int cx = (int)(local % g.n3[0]);
+    const int cy = (int)((local / g.n3[0]) % g.n3[1]);
+    const int cz = (int)(local / (g.n3[0] * g.n3[1]));
+    uint32_t* out = d.cand + s * d.K;
+    for (int dz = -1; dz <= 1; ++dz) {
+      int z = cz + dz;
+      if (g.periodic & 4) z = (z + (int)g.n3[2]) % (int)g.n3[2];
+      if (z < 0 || z >= (int)g.n3[2]) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        int y = cy + dy;
+        if (g.periodic & 2) y = (y + (int)g.n3[1]) % (int)g.n3[1];
+        if (y < 0 || y >= (int)g.n3[1]) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          int x = cx + dx;
+          if (g.periodic & 1) x = (x + (int)g.n3[0]) % (int)g.n3[0];
+          if (x < 0 || x >= (int)g.n3[0]) continue;
+          const uint32_t cell = g.base + ((uint32_t)z * g.n3[1] + (uint32_t)y) * g.n3[0] + (uint32_t)x;
+          const uint32_t t1 = d.cell_end[cell];
+          for (uint32_t t = d.cell_start[cell]; t < t1; ++t) {
+            const uint32_t j = is[t];
+            if (j == i) continue;
+            V3<double> dd = load3(d.pos, n, j) - xi;
+            if (g.periodic) dd = min_image(g, dd);
+            const double rs = ri + d.rc_pf[d.shape[j]];
+            if (!(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z < rs * rs * (1.0 + kPrefilterSlack))) continue;
+            if (cnt < d.K)
+              out[cnt] = j;
+            else
+              over = 1;
+            ++cnt;
+          }
+        }
+      }
+    }
+    d.cand_count[s] = cnt < d.K ? cnt : d.K;
+  }
+  count_add(&d.counters[kCntOverflow], over);
+  uint32_t m = wave_max(cnt);
+  if (__lane_id() == 0 && m) atomicMax((unsigned long long*)&d.counters[kCntMaxCand], (unsigned long long)m);
+}
+
+void launch_candidates(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
+  hipLaunchKernelGGL(k_candidates, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d,
+                     keys_s, idx_s);
+}
+
+// flat entry list: entry e = entry_off[s] + k holds (idx_s[s], cand[s*K + k])
+__global__ void k_expand(uint64_t n, uint32_t K, const uint32_t* __restrict__ is,
+                         const uint32_t* __restrict__ cand, const uint32_t* __restrict__ cnt,
+                         const uint32_t* __restrict__ off, uint32_t* __restrict__ pi,
+                         uint32_t* __restrict__ pj, unsigned long long* __restrict__ counters) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t c = 0;
+  if (s < n) {
+    c = cnt[s];
+    const uint32_t i = is[s];
+    const uint32_t e0 = off[s];
+    for (uint32_t k = 0; k < c; ++k) {
+      pi[e0 + k] = i;
+      pj[e0 + k] = cand[s * K + k];
+    }
+  }
+  count_add(&counters[kCntCandidates], c);
+}
+
+void launch_expand(const DevScene& d, const uint32_t* idx_s, hipStream_t s) {
+  hipLaunchKernelGGL(k_expand, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d.n, d.K,
+                     idx_s, d.cand, d.cand_count, d.entry_off, d.pair_i, d.pair_j, d.counters);
+}
+
+int persistent_blocks() {
+  static int nb = 0;
+  if (!nb) {
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    nb = (cus > 0 ? cus : 256) * 8;
+  }
+  return nb;
+}
+
+}  // namespace gr

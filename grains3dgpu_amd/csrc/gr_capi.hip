@@ -1,0 +1,1096 @@
+// gr_capi.hip -- the C-ABI of include/grains_hip.h: scene set-up on the host
+// (mass properties, linked-cell layout), device memory, and the step
+// pipeline.  Replaces ComponentManagerGPU (Grains/Component/src/
+// ComponentManagerGPU.cpp) and the GrainsGPU driver loop
+// (Grains/Base/src/GrainsGPU.cpp:90-163).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/grains_hip.h"
+#include "gr_internal.hpp"
+
+using namespace gr;
+
+namespace {
+
+constexpr double kPi = 3.1415926535897932385;
+constexpr double kEps = 1.e-10;
+
+// ---- host mass properties (RigidBody ctor, RigidBody.cpp:19-59) -------------
+struct HostShape {
+  grains_shape_t s;
+  double volume = 0, mass = 0, inertia[6] = {0}, inv[6] = {0};
+  double rc = 0, rc_exact = 0, bbox[3] = {0}, core[5] = {0};
+  std::vector<double> verts, core_verts;
+};
+
+double beta_fn(double x, double y) { return std::exp(std::lgamma(x) + std::lgamma(y) - std::lgamma(x + y)); }
+
+// Superquadric::computeCircumscribedRadius (Superquadric.cpp:122-155)
+double sq_rc_reference(double a, double b, double c, double n1, double n2) {
+  if (n1 == 2.0 && n2 == 2.0) return std::max(a, std::max(b, c));
+  if (n1 == 2.0) {
+    double al = std::pow(b / a, 2.0 / (n2 - 2.0));
+    double xt = 1.0 / std::pow(1.0 + std::pow(al, n2), 1.0 / n2);
+    return std::max(c, std::sqrt(a * xt * a * xt + al * b * xt * al * b * xt));
+  }
+  if (n2 == 2.0) {
+    double m = std::max(a, b);
+    double be = std::pow(c * c / (m * m), 1.0 / (n1 - 2.0));
+    double xt = 1.0 / std::pow(1.0 + std::pow(be, n1), 1.0 / n1);
+    return std::sqrt(m * xt * m * xt + be * c * xt * be * c * xt);
+  }
+  double al = std::pow(b / a, 2.0 / (n2 - 2.0));
+  double ga = std::pow(1.0 + std::pow(al, n2), n1 / n2 - 1.0);
+  double be = std::pow(ga * c * c / (a * a), 1.0 / (n1 - 1.0));
+  double xt = 1.0 / std::pow(std::pow(1.0 + std::pow(al, n2), n1 / n2) + std::pow(be, n1), 1.0 / n1);
+  return std::sqrt(a * xt * a * xt + al * b * xt * al * b * xt + be * c * xt * be * c * xt);
+}
+
+// True circumscribed radius: largest |p| over the critical points of every
+// coordinate family of (|x/a|^n2 + |y/b|^n2)^(n1/n2) + |z/c|^n1 = 1.
+double planar_max(double a, double b, double n) {
+  double r = std::max(a, b);
+  if (n > 2.0) {
+    double t = std::pow(b / a, n / (n - 2.0));
+    double x = a * std::pow(1.0 + std::pow(t * a / b, n), -1.0 / n);
+    r = std::max(r, std::sqrt(x * x + t * x * t * x));
+  }
+  return r;
+}
+double sq_rc_exact(double a, double b, double c, double n1, double n2) {
+  double r = std::max(planar_max(a, b, n2), std::max(planar_max(a, c, n1), planar_max(b, c, n1)));
+  if (n1 > 2.0 && n2 > 2.0) {
+    double al = std::pow(b / a, 2.0 / (n2 - 2.0));
+    double T = 1.0 + std::pow(al, n2);
+    double be = std::pow(c * c / (a * a) * std::pow(T, n1 / n2 - 1.0), 1.0 / (n1 - 2.0));
+    double xt = std::pow(std::pow(T, n1 / n2) + std::pow(be, n1), -1.0 / n1);
+    double x = a * xt, y = al * b * xt, z = be * c * xt;
+    r = std::max(r, std::sqrt(x * x + y * y + z * z));
+  }
+  return r;
+}
+
+// Convex hull volume, centroid and inertia (unit density, about the
+// centroid): supporting planes through three points are facets, their
+// coplanar points fan-triangulated around the facet centre.
+void hull_mass(const std::vector<double>& P, double& vol, double* I6) {
+  const int n = (int)P.size() / 3;
+  auto pt = [&](int i) { return V3<double>{P[3 * i], P[3 * i + 1], P[3 * i + 2]}; };
+  double V0 = 0, c[3] = {0, 0, 0}, acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j)
+      for (int k = j + 1; k < n; ++k) {
+        V3<double> a = pt(i), b = pt(j), cc = pt(k);
+        V3<double> nr = cross(b - a, cc - a);
+        double nn = norm(nr);
+        if (nn == 0.0) continue;
+        int pos = 0, neg = 0;
+        bool first = true;
+        std::vector<int> on;
+        for (int l = 0; l < n; ++l) {
+          double sd = dot(nr, pt(l) - a);
+          if (sd > 1e-12 * nn) ++pos;
+          else if (sd < -1e-12 * nn) ++neg;
+          else {
+            on.push_back(l);
+            if (l < k && l != i && l != j) first = false;
+          }
+        }
+        if ((pos && neg) || !first) continue;
+        V3<double> out = pos ? -nr : nr;
+        V3<double> ctr = {0, 0, 0};
+        for (int l : on) ctr = ctr + pt(l);
+        ctr = (1.0 / on.size()) * ctr;
+        V3<double> e1 = a - ctr;
+        e1 = (1.0 / norm(e1)) * e1;
+        V3<double> e2 = cross((1.0 / nn) * out, e1);
+        std::vector<std::pair<double, int>> ang;
+        for (int l : on) {
+          V3<double> dd = pt(l) - ctr;
+          ang.push_back({std::atan2(dot(dd, e2), dot(dd, e1)), l});
+        }
+        std::sort(ang.begin(), ang.end());
+        for (size_t t = 1; t + 1 < ang.size(); ++t) {
+          V3<double> p0 = pt(ang[0].second), p1 = pt(ang[t].second), p2 = pt(ang[t + 1].second);
+          double dv = dot(p0, cross(p1, p2)) / 6.0;
+          V0 += dv;
+          const double Q[3][3] = {{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, {p2.x, p2.y, p2.z}};
+          for (int q = 0; q < 3; ++q) c[q] += dv * (Q[0][q] + Q[1][q] + Q[2][q]) / 4.0;
+          const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+          for (int q = 0; q < 6; ++q) {
+            double s1 = 0, si = 0, sj = 0;
+            for (int r = 0; r < 3; ++r) {
+              s1 += Q[r][ii[q]] * Q[r][jj[q]];
+              si += Q[r][ii[q]];
+              sj += Q[r][jj[q]];
+            }
+            acc[q] += dv / 20.0 * (s1 + si * sj);
+          }
+        }
+      }
+  vol = V0;
+  for (int q = 0; q < 3; ++q) c[q] /= V0;
+  double Sxx = acc[0] - V0 * c[0] * c[0], Syy = acc[1] - V0 * c[1] * c[1], Szz = acc[2] - V0 * c[2] * c[2];
+  double Sxy = acc[3] - V0 * c[0] * c[1], Sxz = acc[4] - V0 * c[0] * c[2], Syz = acc[5] - V0 * c[1] * c[2];
+  I6[0] = Syy + Szz;
+  I6[1] = -Sxy;
+  I6[2] = -Sxz;
+  I6[3] = Sxx + Szz;
+  I6[4] = -Syz;
+  I6[5] = Sxx + Syy;
+}
+
+void sym_inv(const double* I, double* r) {
+  double a = I[0], b = I[1], c = I[2], d = I[3], e = I[4], f = I[5];
+  double A = d * f - e * e, B = -(b * f - c * e), C = b * e - c * d;
+  double det = a * A + b * B + c * C;
+  r[0] = A / det;
+  r[1] = B / det;
+  r[2] = C / det;
+  r[3] = (a * f - c * c) / det;
+  r[4] = -(a * e - b * c) / det;
+  r[5] = (a * d - b * b) / det;
+}
+
+int derive_shape(HostShape& h, const grains_shape_t& s, const double* verts, uint32_t nverts) {
+  h.s = s;
+  const double* d = s.dims;
+  double I[6] = {0, 0, 0, 0, 0, 0}, Ii[6] = {0, 0, 0, 0, 0, 0};
+  switch (s.kind) {
+    case GRAINS_SPHERE: {  // Sphere.cpp:72-105
+      double r = d[0];
+      if (!(r > 0)) return GRAINS_EINVAL;
+      h.volume = 4.0 * kPi * r * r * r / 3.0;
+      double r5 = r * r * r * r * r;
+      I[5] = I[3] = I[0] = 8.0 * kPi / 15.0 * r5;
+      Ii[5] = Ii[3] = Ii[0] = 1.0 / I[0];
+      h.rc = r;
+      h.bbox[0] = h.bbox[1] = h.bbox[2] = r;
+      break;
+    }
+    case GRAINS_BOX: {  // Box.cpp:81-123
+      double ex = d[0], ey = d[1], ez = d[2];
+      if (!(ex > 0 && ey > 0 && ez > 0)) return GRAINS_EINVAL;
+      h.volume = 8.0 * ex * ey * ez;
+      I[0] = 8.0 * ex * ey * ez * (ey * ey + ez * ez) / 3.0;
+      I[3] = 8.0 * ex * ey * ez * (ex * ex + ez * ez) / 3.0;
+      I[5] = 8.0 * ex * ey * ez * (ey * ey + ex * ex) / 3.0;
+      Ii[0] = 1.0 / I[0];
+      Ii[3] = 1.0 / I[3];
+      Ii[5] = 1.0 / I[5];
+      h.rc = std::sqrt(ex * ex + ey * ey + ez * ez);
+      h.bbox[0] = ex, h.bbox[1] = ey, h.bbox[2] = ez;
+      break;
+    }
+    case GRAINS_SUPERQUADRIC: {  // Superquadric.cpp:81-163
+      double a = d[0], b = d[1], c = d[2], n1 = d[3], n2 = d[4];
+      if (!(a > 0 && b > 0 && c > 0 && n1 >= 2 && n2 >= 2)) return GRAINS_EINVAL;
+      double e1 = 2.0 / n1, e2 = 2.0 / n2;
+      h.volume = 2.0 / 3.0 * a * b * c * e1 * e2 * beta_fn(e1, 0.5 * e1) * beta_fn(0.5 * e2, 0.5 * e2);
+      double Ci = 0.4 * a * b * c * e1 * e2;
+      double p1 = beta_fn(1.5 * e2, 0.5 * e2) * beta_fn(2.0 * e1, 0.5 * e1);
+      double p2 = c * c * beta_fn(0.5 * e2, 0.5 * e2) * beta_fn(1.5 * e1, e1);
+      I[0] = Ci * (b * b * p1 + p2);
+      I[3] = Ci * (a * a * p1 + p2);
+      I[5] = Ci * (a * a + b * b) * p1;
+      Ii[0] = 1.0 / I[0];
+      Ii[3] = 1.0 / I[3];
+      Ii[5] = 1.0 / I[5];
+      h.rc = sq_rc_reference(a, b, c, n1, n2);
+      h.rc_exact = sq_rc_exact(a, b, c, n1, n2);
+      h.bbox[0] = a, h.bbox[1] = b, h.bbox[2] = c;
+      break;
+    }
+    case GRAINS_RECTANGLE: {  // Rectangle.cpp:72-109
+      double LX = d[0], LY = d[1];
+      h.volume = 4.0 * LX * LY;
+      I[0] = 4.0 / 3.0 * LX * LY * LY * LY;
+      I[3] = 4.0 / 3.0 * LX * LX * LX * LY;
+      I[5] = 2.0 / 3.0 * LX * LY * (LX * LX + LY * LY);
+      Ii[0] = 1.0 / I[0];
+      Ii[3] = 1.0 / I[3];
+      Ii[5] = 1.0 / I[5];
+      h.rc = std::sqrt(LX * LX + LY * LY);
+      h.bbox[0] = LX, h.bbox[1] = LY, h.bbox[2] = 0.0;
+      break;
+    }
+    case GRAINS_POLYHEDRON: {
+      if (!verts || s.n_vertices < 4 || s.vertex_offset < 0 ||
+          (uint64_t)s.vertex_offset + (uint64_t)s.n_vertices > nverts)
+        return GRAINS_EINVAL;
+      h.verts.assign(verts + 3 * (size_t)s.vertex_offset, verts + 3 * (size_t)(s.vertex_offset + s.n_vertices));
+      hull_mass(h.verts, h.volume, I);
+      if (!(h.volume > 0)) return GRAINS_EINVAL;
+      sym_inv(I, Ii);
+      for (int v = 0; v < s.n_vertices; ++v) {
+        V3<double> p = {h.verts[3 * v], h.verts[3 * v + 1], h.verts[3 * v + 2]};
+        h.rc = std::max(h.rc, norm(p));
+        h.bbox[0] = std::max(h.bbox[0], std::fabs(p.x));
+        h.bbox[1] = std::max(h.bbox[1], std::fabs(p.y));
+        h.bbox[2] = std::max(h.bbox[2], std::fabs(p.z));
+      }
+      break;
+    }
+    default:
+      return GRAINS_EUNSUPPORTED;
+  }
+  if (s.kind != GRAINS_SUPERQUADRIC) h.rc_exact = h.rc;
+  h.mass = s.density * h.volume;
+  for (int i = 0; i < 6; ++i) {
+    h.inertia[i] = s.density == 0 ? 0.0 : I[i] * s.density;
+    h.inv[i] = s.density == 0 ? 0.0 : Ii[i] / s.density;
+  }
+  // crust core: RigidBody::m_scaling (RigidBody.cpp:27-31)
+  std::copy(d, d + 5, h.core);
+  switch (s.kind) {
+    case GRAINS_SPHERE: h.core[0] = d[0] - s.crust; break;
+    case GRAINS_BOX:
+    case GRAINS_SUPERQUADRIC:
+      for (int k = 0; k < 3; ++k) h.core[k] = d[k] - s.crust;
+      break;
+    case GRAINS_POLYHEDRON:
+      h.core_verts = h.verts;
+      for (size_t v = 0; v < h.verts.size(); ++v) {
+        int k = v % 3;
+        h.core_verts[v] *= h.bbox[k] > 0 ? (h.bbox[k] - s.crust) / h.bbox[k] : 1.0;
+      }
+      break;
+    default: break;
+  }
+  return GRAINS_OK;
+}
+
+template <class T>
+ShapeNP<T> to_np(const HostShape& h, bool quirks, int32_t voff) {
+  ShapeNP<T> r;
+  std::memset(&r, 0, sizeof(r));
+  r.kind = h.s.kind;
+  r.material = h.s.material;
+  r.nv = h.s.n_vertices;
+  r.voff = voff;
+  for (int k = 0; k < 5; ++k) {
+    r.dims[k] = T(h.s.dims[k]);
+    r.core[k] = T(h.core[k]);
+  }
+  r.crust = T(h.s.crust);
+  r.rc = T(quirks ? h.rc : h.rc_exact);
+  for (int k = 0; k < 3; ++k) r.bbox[k] = T(h.bbox[k]);
+  r.mass = T(h.mass);
+  return r;
+}
+
+}  // namespace
+
+// ---- context ------------------------------------------------------------------
+struct grains_ctx {
+  grains_config_t cfg;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<HostShape> shapes;
+  std::vector<double> verts_in;
+  std::vector<grains_hooke_t> tables;
+  uint32_t nmat = 0, ntables = 0;
+  std::vector<grains_sim_t> sims;
+  std::vector<uint32_t> obs_shape, obs_sim;
+  std::vector<double> obs_tr;
+  std::vector<uint32_t> pshape, psim;
+  uint64_t n = 0;
+  bool layout_ok = false;
+  DevScene d;
+  std::vector<void*> allocs;       // layout-dependent buffers
+  std::vector<void*> state_allocs; // particle state
+  int passes = 1;
+  uint32_t *keys_s = nullptr, *idx_s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  grains_stats_t stats;
+  std::vector<SimGrid> grids;
+};
+
+#define GR_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                     \
+      return GRAINS_EDEVICE;                                                            \
+    }                                                                                   \
+  } while (0)
+
+static int fail(grains_ctx* ctx, int code, const std::string& msg) {
+  ctx->err = msg;
+  return code;
+}
+
+template <class P>
+static int dalloc(grains_ctx* ctx, P** p, size_t bytes, std::vector<void*>& list) {
+  void* q = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, GRAINS_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  list.push_back(q);
+  *p = (P*)q;
+  return GRAINS_OK;
+}
+
+static void free_list(std::vector<void*>& list) {
+  for (void* p : list) (void)hipFree(p);
+  list.clear();
+}
+
+static void drop_graph(grains_ctx* ctx) {
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+}
+
+extern "C" const char* grains_version(void) { return "grains_hip 0.1 gfx950"; }
+
+extern "C" int grains_create(const grains_config_t* cfg, grains_ctx_t** out) {
+  if (!cfg || !out) return GRAINS_EINVAL;
+  *out = nullptr;
+  if (!(cfg->dt > 0) || (cfg->precision != GRAINS_FP64 && cfg->precision != GRAINS_FP32_MIXED))
+    return GRAINS_EINVAL;
+  grains_ctx* ctx = new (std::nothrow) grains_ctx();
+  if (!ctx) return GRAINS_ENOMEM;
+  ctx->cfg = *cfg;
+  if (ctx->cfg.max_neighbors <= 0) ctx->cfg.max_neighbors = 48;
+  if (ctx->cfg.block_size <= 0) ctx->cfg.block_size = 256;
+  std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+  std::memset(&ctx->d, 0, sizeof(ctx->d));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    delete ctx;
+    return GRAINS_EDEVICE;
+  }
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    delete ctx;
+    return GRAINS_EDEVICE;
+  }
+  *out = ctx;
+  return GRAINS_OK;
+}
+
+extern "C" void grains_destroy(grains_ctx_t* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
+  free_list(ctx->allocs);
+  free_list(ctx->state_allocs);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" const char* grains_last_error(const grains_ctx_t* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int grains_set_shapes(grains_ctx_t* ctx, const grains_shape_t* shapes, uint32_t n_shapes,
+                                 const double* vertices, uint32_t n_vertices) {
+  if (!ctx || (!shapes && n_shapes)) return GRAINS_EINVAL;
+  std::vector<HostShape> hs(n_shapes);
+  for (uint32_t i = 0; i < n_shapes; ++i) {
+    int rc = derive_shape(hs[i], shapes[i], vertices, n_vertices);
+    if (rc) return fail(ctx, rc, "shape " + std::to_string(i) + " is invalid or unsupported");
+  }
+  ctx->shapes.swap(hs);
+  ctx->verts_in.assign(vertices, vertices + 3 * (size_t)n_vertices);
+  ctx->layout_ok = false;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_set_contact_tables(grains_ctx_t* ctx, const grains_hooke_t* t, uint32_t n_materials,
+                                         uint32_t n_tables) {
+  if (!ctx || !t || !n_materials || !n_tables) return GRAINS_EINVAL;
+  const size_t per = (size_t)n_materials * (n_materials + 1) / 2;
+  for (size_t i = 0; i < per * n_tables; ++i)
+    if (!(t[i].kn > 0) || !(t[i].en > 0 && t[i].en <= 1))
+      return fail(ctx, GRAINS_EINVAL, "contact table entry " + std::to_string(i) + ": need kn > 0, 0 < en <= 1");
+  ctx->tables.assign(t, t + per * n_tables);
+  ctx->nmat = n_materials;
+  ctx->ntables = n_tables;
+  ctx->layout_ok = false;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_set_sims(grains_ctx_t* ctx, const grains_sim_t* sims, uint32_t n_sims) {
+  if (!ctx || !sims || !n_sims) return GRAINS_EINVAL;
+  ctx->sims.assign(sims, sims + n_sims);
+  ctx->layout_ok = false;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_set_obstacles(grains_ctx_t* ctx, const uint32_t* shape, const uint32_t* sim,
+                                    const double* transforms, uint32_t n) {
+  if (!ctx || (n && (!shape || !transforms))) return GRAINS_EINVAL;
+  ctx->obs_shape.assign(shape, shape + n);
+  ctx->obs_sim.assign(n, 0u);
+  if (sim) ctx->obs_sim.assign(sim, sim + n);
+  ctx->obs_tr.assign(transforms, transforms + 12 * (size_t)n);
+  ctx->layout_ok = false;
+  return GRAINS_OK;
+}
+
+static int alloc_state(grains_ctx* ctx) {
+  free_list(ctx->state_allocs);
+  const uint64_t n = ctx->n;
+  DevScene& d = ctx->d;
+  auto& L = ctx->state_allocs;
+  int rc;
+  if ((rc = dalloc(ctx, &d.pos, 3 * n * sizeof(double), L)) || (rc = dalloc(ctx, &d.quat, 4 * n * sizeof(double), L)) ||
+      (rc = dalloc(ctx, &d.vel, 3 * n * sizeof(double), L)) || (rc = dalloc(ctx, &d.omega, 3 * n * sizeof(double), L)) ||
+      (rc = dalloc(ctx, &d.shape, n * sizeof(uint32_t), L)) || (rc = dalloc(ctx, &d.sim, n * sizeof(uint32_t), L)) ||
+      (rc = dalloc(ctx, &d.contact_count, n * sizeof(int32_t), L)) ||
+      (rc = dalloc(ctx, &d.last_force, 3 * n * sizeof(double), L)) ||
+      (rc = dalloc(ctx, &d.last_torque, 3 * n * sizeof(double), L)))
+    return rc;
+  std::vector<double> zero(4 * n, 0.0), q(4 * n, 0.0);
+  for (uint64_t i = 0; i < n; ++i) q[3 * n + i] = 1.0;
+  GR_HIP(hipMemcpy(d.pos, zero.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d.vel, zero.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d.omega, zero.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d.last_force, zero.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d.last_torque, zero.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d.quat, q.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d.contact_count, zero.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d.shape, ctx->pshape.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d.sim, ctx->psim.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  return GRAINS_OK;
+}
+
+extern "C" int grains_set_particles(grains_ctx_t* ctx, const uint32_t* shape, const uint32_t* sim, uint64_t n) {
+  if (!ctx || !shape || !n) return GRAINS_EINVAL;
+  if (n >= (1ull << 31)) return fail(ctx, GRAINS_EINVAL, "too many particles");
+  if (ctx->shapes.empty() || ctx->sims.empty())
+    return fail(ctx, GRAINS_ESTATE, "set shapes and sims before particles");
+  for (uint64_t i = 0; i < n; ++i) {
+    if (shape[i] >= ctx->shapes.size()) return fail(ctx, GRAINS_EINVAL, "particle shape index out of range");
+    int k = ctx->shapes[shape[i]].s.kind;
+    if (k == GRAINS_RECTANGLE) return fail(ctx, GRAINS_EINVAL, "a particle cannot be a Rectangle");
+    if (ctx->shapes[shape[i]].mass <= 0) return fail(ctx, GRAINS_EINVAL, "particle mass must be > 0");
+    uint32_t sm = sim ? sim[i] : 0u;
+    if (sm >= ctx->sims.size()) return fail(ctx, GRAINS_EINVAL, "particle sim index out of range");
+    if (i && sim && sim[i] < sim[i - 1]) return fail(ctx, GRAINS_EINVAL, "particles must be grouped by ascending sim");
+  }
+  ctx->n = n;
+  ctx->pshape.assign(shape, shape + n);
+  ctx->psim.assign(n, 0u);
+  if (sim) ctx->psim.assign(sim, sim + n);
+  ctx->d.n = n;
+  ctx->layout_ok = false;
+  drop_graph(ctx);
+  free_list(ctx->allocs);
+  return alloc_state(ctx);
+}
+
+// cell layout, obstacle ranges, shape tables and work buffers
+static int finalize(grains_ctx* ctx) {
+  if (ctx->layout_ok) return GRAINS_OK;
+  if (!ctx->n) return fail(ctx, GRAINS_ESTATE, "no particles");
+  if (ctx->tables.empty()) return fail(ctx, GRAINS_ESTATE, "no contact table");
+  drop_graph(ctx);
+  free_list(ctx->allocs);
+  DevScene& d = ctx->d;
+  const bool quirks = (ctx->cfg.flags & GRAINS_REF_QUIRKS) != 0;
+  const uint32_t ns = (uint32_t)ctx->sims.size();
+  for (uint32_t s : ctx->obs_shape)
+    if (s >= ctx->shapes.size() || ctx->shapes[s].s.kind != GRAINS_RECTANGLE)
+      return fail(ctx, GRAINS_EUNSUPPORTED, "obstacles must be Rectangle walls");
+  for (uint32_t s : ctx->obs_sim)
+    if (s >= ns) return fail(ctx, GRAINS_EINVAL, "obstacle sim index out of range");
+  for (const auto& h : ctx->shapes)
+    if ((uint32_t)h.s.material >= ctx->nmat) return fail(ctx, GRAINS_EINVAL, "material id beyond the contact table");
+  // obstacles grouped by sim (stable, so their order within a sim is kept)
+  std::vector<uint32_t> ord(ctx->obs_shape.size());
+  for (size_t i = 0; i < ord.size(); ++i) ord[i] = (uint32_t)i;
+  std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return ctx->obs_sim[a] < ctx->obs_sim[b]; });
+  std::vector<uint32_t> osh(ord.size());
+  std::vector<double> otr(12 * ord.size());
+  for (size_t k = 0; k < ord.size(); ++k) {
+    osh[k] = ctx->obs_shape[ord[k]];
+    std::copy(&ctx->obs_tr[12 * ord[k]], &ctx->obs_tr[12 * ord[k]] + 12, &otr[12 * k]);
+  }
+  // linked cells per sim (Grains.cpp:226-247, LinkedCell.cpp:16-29)
+  ctx->grids.assign(ns, SimGrid());
+  std::vector<double> lc(ns, 0.0);
+  for (uint64_t i = 0; i < ctx->n; ++i) {
+    const HostShape& h = ctx->shapes[ctx->pshape[i]];
+    lc[ctx->psim[i]] = std::max(lc[ctx->psim[i]], quirks ? h.rc : h.rc_exact);
+  }
+  uint64_t base = 0;
+  const uint32_t per = ctx->nmat * (ctx->nmat + 1) / 2;
+  for (uint32_t s = 0; s < ns; ++s) {
+    const grains_sim_t& S = ctx->sims[s];
+    SimGrid& g = ctx->grids[s];
+    double h = S.cell_size;
+    if (!(h > 0)) h = std::max(1.0, S.cell_size_factor) * 2.0 * lc[s];
+    if (!(h > 0)) return fail(ctx, GRAINS_EINVAL, "sim " + std::to_string(s) + ": zero cell size");
+    g.h = h;
+    for (int k = 0; k < 3; ++k) {
+      g.lo[k] = S.domain_min[k];
+      g.hi[k] = S.domain_max[k];
+      g.L[k] = S.domain_max[k] - S.domain_min[k];
+      g.g[k] = S.gravity[k];
+      double c = kEps;
+      c += S.domain_max[k] - S.domain_min[k];
+      c /= h;
+      if (!(c >= 1.0) || c > 4e9) return fail(ctx, GRAINS_EINVAL, "sim " + std::to_string(s) + ": bad domain");
+      g.n3[k] = (uint32_t)c;
+      if (((S.periodic >> k) & 1) && g.n3[k] < 3)
+        return fail(ctx, GRAINS_EINVAL, "periodic direction needs >= 3 cells");
+    }
+    g.periodic = S.periodic;
+    if (S.contact_table < 0 || (uint32_t)S.contact_table >= ctx->ntables)
+      return fail(ctx, GRAINS_EINVAL, "contact table index out of range");
+    g.table = (uint32_t)S.contact_table * per;
+    g.base = (uint32_t)base;
+    base += (uint64_t)g.n3[0] * g.n3[1] * g.n3[2];
+    if (base >= (1ull << 32) - 1) return fail(ctx, GRAINS_EINVAL, "too many cells");
+    g.obs_begin = g.obs_end = 0;
+  }
+  for (size_t k = 0; k < ord.size(); ++k) {
+    SimGrid& g = ctx->grids[ctx->obs_sim[ord[k]]];
+    if (g.obs_end == g.obs_begin) g.obs_begin = (uint32_t)k;
+    g.obs_end = (uint32_t)k + 1;
+  }
+  d.ncells = (uint32_t)base;
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < base) ++bits;
+  ctx->passes = (bits + 7) / 8;
+  // shape tables
+  const size_t S = ctx->shapes.size();
+  std::vector<ShapeNP<double>> np64(S);
+  std::vector<ShapeNP<float>> np32(S);
+  std::vector<ShapeDyn> dyn(S);
+  std::vector<double> rcpf(S), v64, cv64;
+  for (size_t i = 0; i < S; ++i) {
+    const HostShape& h = ctx->shapes[i];
+    int32_t voff = (int32_t)(v64.size() / 3);
+    np64[i] = to_np<double>(h, quirks, voff);
+    np32[i] = to_np<float>(h, quirks, voff);
+    dyn[i].mass = h.mass;
+    std::copy(h.inertia, h.inertia + 6, dyn[i].inertia);
+    std::copy(h.inv, h.inv + 6, dyn[i].inv);
+    dyn[i].pad = 0;
+    rcpf[i] = quirks ? h.rc : h.rc_exact;
+    v64.insert(v64.end(), h.verts.begin(), h.verts.end());
+    cv64.insert(cv64.end(), h.core_verts.begin(), h.core_verts.end());
+  }
+  if (v64.empty()) v64.assign(3, 0.0), cv64.assign(3, 0.0);
+  std::vector<float> v32(v64.begin(), v64.end()), cv32(cv64.begin(), cv64.end());
+  std::vector<HookeP<double>> hk64(ctx->tables.size());
+  std::vector<HookeP<float>> hk32(ctx->tables.size());
+  for (size_t i = 0; i < ctx->tables.size(); ++i) {
+    const grains_hooke_t& t = ctx->tables[i];
+    double muen = std::log(t.en) / std::sqrt(kPi * kPi + std::log(t.en) * std::log(t.en));
+    hk64[i] = {t.kn, t.en, t.etat, t.muc, t.kr, muen};
+    hk32[i] = {(float)t.kn, (float)t.en, (float)t.etat, (float)t.muc, (float)t.kr, (float)muen};
+  }
+  auto& L = ctx->allocs;
+  const uint64_t n = ctx->n;
+  const uint32_t K = (uint32_t)ctx->cfg.max_neighbors;
+  const uint64_t ecap = n * K;
+  const uint64_t ntiles = (n + kTile - 1) / kTile;
+  const uint64_t nblk = std::max((n + kTile - 1) / kTile, (256 * ntiles + kTile - 1) / kTile) + 2;
+  const size_t res_sz = ctx->cfg.precision == GRAINS_FP64 ? sizeof(PairRes<double>) : sizeof(PairRes<float>);
+  int rc;
+  ShapeNP<double>* p64;
+  ShapeNP<float>* p32;
+  ShapeDyn* pdyn;
+  double *prc, *pv64, *pcv64, *potr;
+  float *pv32, *pcv32;
+  HookeP<double>* ph64;
+  HookeP<float>* ph32;
+  SimGrid* pgr;
+  uint32_t* posh;
+  if ((rc = dalloc(ctx, &p64, S * sizeof(ShapeNP<double>), L)) || (rc = dalloc(ctx, &p32, S * sizeof(ShapeNP<float>), L)) ||
+      (rc = dalloc(ctx, &pdyn, S * sizeof(ShapeDyn), L)) || (rc = dalloc(ctx, &prc, S * sizeof(double), L)) ||
+      (rc = dalloc(ctx, &pv64, v64.size() * sizeof(double), L)) || (rc = dalloc(ctx, &pcv64, cv64.size() * sizeof(double), L)) ||
+      (rc = dalloc(ctx, &pv32, v32.size() * sizeof(float), L)) || (rc = dalloc(ctx, &pcv32, cv32.size() * sizeof(float), L)) ||
+      (rc = dalloc(ctx, &ph64, hk64.size() * sizeof(HookeP<double>), L)) ||
+      (rc = dalloc(ctx, &ph32, hk32.size() * sizeof(HookeP<float>), L)) ||
+      (rc = dalloc(ctx, &pgr, ns * sizeof(SimGrid), L)) || (rc = dalloc(ctx, &posh, (osh.size() + 1) * sizeof(uint32_t), L)) ||
+      (rc = dalloc(ctx, &potr, (otr.size() + 12) * sizeof(double), L)) ||
+      (rc = dalloc(ctx, &d.key, n * 4, L)) || (rc = dalloc(ctx, &d.keyA, n * 4, L)) || (rc = dalloc(ctx, &d.idxA, n * 4, L)) ||
+      (rc = dalloc(ctx, &d.keyB, n * 4, L)) || (rc = dalloc(ctx, &d.idxB, n * 4, L)) || (rc = dalloc(ctx, &d.rank, n * 4, L)) ||
+      (rc = dalloc(ctx, &d.cell_start, (size_t)d.ncells * 4, L)) || (rc = dalloc(ctx, &d.cell_end, (size_t)d.ncells * 4, L)) ||
+      (rc = dalloc(ctx, &d.hist, (256 * ntiles + 1) * 4, L)) || (rc = dalloc(ctx, &d.scan_blocks, nblk * 4, L)) ||
+      (rc = dalloc(ctx, &d.cand, ecap * 4, L)) || (rc = dalloc(ctx, &d.cand_count, n * 4, L)) ||
+      (rc = dalloc(ctx, &d.entry_off, (n + 1) * 4, L)) || (rc = dalloc(ctx, &d.pair_i, ecap * 4, L)) ||
+      (rc = dalloc(ctx, &d.pair_j, ecap * 4, L)) || (rc = dalloc(ctx, &d.pair_flag, ecap * 4, L)) ||
+      (rc = dalloc(ctx, &d.pair_res, ecap * res_sz, L)) || (rc = dalloc(ctx, &d.refine_list, ecap * 4, L)) ||
+      (rc = dalloc(ctx, &d.refine_count, 16, L)) || (rc = dalloc(ctx, &d.counters, kNumCounters * 8, L)))
+    return rc;
+  GR_HIP(hipMemcpy(p64, np64.data(), S * sizeof(ShapeNP<double>), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(p32, np32.data(), S * sizeof(ShapeNP<float>), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(pdyn, dyn.data(), S * sizeof(ShapeDyn), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(prc, rcpf.data(), S * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(pv64, v64.data(), v64.size() * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(pcv64, cv64.data(), cv64.size() * sizeof(double), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(pv32, v32.data(), v32.size() * sizeof(float), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(pcv32, cv32.data(), cv32.size() * sizeof(float), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(ph64, hk64.data(), hk64.size() * sizeof(HookeP<double>), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(ph32, hk32.data(), hk32.size() * sizeof(HookeP<float>), hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(pgr, ctx->grids.data(), ns * sizeof(SimGrid), hipMemcpyHostToDevice));
+  if (!osh.empty()) {
+    GR_HIP(hipMemcpy(posh, osh.data(), osh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    GR_HIP(hipMemcpy(potr, otr.data(), otr.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  d.np64 = p64;
+  d.np32 = p32;
+  d.dyn = pdyn;
+  d.rc_pf = prc;
+  d.verts64 = pv64;
+  d.cverts64 = pcv64;
+  d.verts32 = pv32;
+  d.cverts32 = pcv32;
+  d.hooke64 = ph64;
+  d.hooke32 = ph32;
+  d.grids = pgr;
+  d.obs_shape = posh;
+  d.obs_tr = potr;
+  d.nsims = ns;
+  d.nobs = (uint32_t)osh.size();
+  d.nmat = ctx->nmat;
+  d.K = K;
+  d.ecap = ecap;
+  d.dt = ctx->cfg.dt;
+  d.flags = ctx->cfg.flags;
+  d.gjk_rel = ctx->cfg.gjk_rel_tol;
+  d.precision = ctx->cfg.precision;
+  launch_fill_u32(d.cand_count, n, 0u, ctx->stream);
+  launch_fill_u32(d.entry_off, n + 1, 0u, ctx->stream);
+  launch_fill_u32(d.refine_count, 4, 0u, ctx->stream);
+  launch_fill_u32((uint32_t*)d.counters, 2 * kNumCounters, 0u, ctx->stream);
+  GR_HIP(hipGetLastError());
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->layout_ok = true;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_set_state(grains_ctx_t* ctx, const double* pos, const double* quat, const double* vel,
+                                const double* omega, uint64_t n) {
+  if (!ctx) return GRAINS_EINVAL;
+  if (n != ctx->n || !n) return fail(ctx, GRAINS_EINVAL, "particle count mismatch");
+  std::vector<double> buf(4 * n);
+  auto put = [&](const double* src, double* dst, int comps) -> int {
+    if (!src) return GRAINS_OK;
+    for (uint64_t i = 0; i < n; ++i)
+      for (int k = 0; k < comps; ++k) buf[k * n + i] = src[comps * i + k];
+    GR_HIP(hipMemcpy(dst, buf.data(), comps * n * sizeof(double), hipMemcpyHostToDevice));
+    return GRAINS_OK;
+  };
+  int rc;
+  if ((rc = put(pos, ctx->d.pos, 3)) || (rc = put(quat, ctx->d.quat, 4)) || (rc = put(vel, ctx->d.vel, 3)) ||
+      (rc = put(omega, ctx->d.omega, 3)))
+    return rc;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_get_state(grains_ctx_t* ctx, double* pos, double* quat, double* vel, double* omega,
+                                uint64_t n) {
+  if (!ctx) return GRAINS_EINVAL;
+  if (n != ctx->n || !n) return fail(ctx, GRAINS_EINVAL, "particle count mismatch");
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<double> buf(4 * n);
+  auto get = [&](double* dst, const double* src, int comps) -> int {
+    if (!dst) return GRAINS_OK;
+    GR_HIP(hipMemcpy(buf.data(), src, comps * n * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i)
+      for (int k = 0; k < comps; ++k) dst[comps * i + k] = buf[k * n + i];
+    return GRAINS_OK;
+  };
+  int rc;
+  if ((rc = get(pos, ctx->d.pos, 3)) || (rc = get(quat, ctx->d.quat, 4)) || (rc = get(vel, ctx->d.vel, 3)) ||
+      (rc = get(omega, ctx->d.omega, 3)))
+    return rc;
+  return GRAINS_OK;
+}
+
+// Transform3 buffers <-> (origin, quaternion).  The quaternion of a basis is
+// taken with the four-branch rule of Quaternion::setQuaternion(Matrix3)
+// (Quaternion.cpp:125-169).
+extern "C" int grains_set_transforms(grains_ctx_t* ctx, const double* t, uint64_t n) {
+  if (!ctx || !t) return GRAINS_EINVAL;
+  if (n != ctx->n) return fail(ctx, GRAINS_EINVAL, "particle count mismatch");
+  std::vector<double> pos(3 * n), q(4 * n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const double* b = t + 12 * i;
+    enum { XX, XY, XZ, YX, YY, YZ, ZX, ZY, ZZ };
+    double qx = 0, qy = 0, qz = 0, qw = 1, den;
+    if (b[YY] > -b[ZZ] && b[XX] > -b[YY] && b[XX] > -b[ZZ]) {
+      den = std::pow(1.0 + b[XX] + b[YY] + b[ZZ], 0.5);
+      qw = 0.5 * den;
+      qx = 0.5 * (b[ZY] - b[YZ]) / den;
+      qy = 0.5 * (b[XZ] - b[ZX]) / den;
+      qz = 0.5 * (b[YX] - b[XY]) / den;
+    } else if (b[YY] < -b[ZZ] && b[XX] > b[YY] && b[XX] > b[ZZ]) {
+      den = std::pow(1.0 + b[XX] - b[YY] - b[ZZ], 0.5);
+      qw = 0.5 * (b[ZY] - b[YZ]) / den;
+      qx = 0.5 * den;
+      qy = 0.5 * (b[XY] + b[YX]) / den;
+      qz = 0.5 * (b[ZX] + b[XZ]) / den;
+    } else if (b[YY] > b[ZZ] && b[XX] < b[YY] && b[XX] < -b[ZZ]) {
+      den = std::pow(1.0 - b[XX] + b[YY] - b[ZZ], 0.5);
+      qw = 0.5 * (b[XZ] - b[ZX]) / den;
+      qx = 0.5 * (b[XY] + b[YX]) / den;
+      qy = 0.5 * den;
+      qz = 0.5 * (b[YZ] + b[ZY]) / den;
+    } else if (b[YY] < b[ZZ] && b[XX] < -b[YY] && b[XX] < b[ZZ]) {
+      den = std::pow(1.0 - b[XX] - b[YY] + b[ZZ], 0.5);
+      qw = 0.5 * (b[YX] - b[XY]) / den;
+      qx = 0.5 * (b[ZX] + b[XZ]) / den;
+      qy = 0.5 * (b[YZ] + b[ZY]) / den;
+      qz = 0.5 * den;
+    }
+    q[4 * i] = qx, q[4 * i + 1] = qy, q[4 * i + 2] = qz, q[4 * i + 3] = qw;
+    pos[3 * i] = b[9], pos[3 * i + 1] = b[10], pos[3 * i + 2] = b[11];
+  }
+  return grains_set_state(ctx, pos.data(), q.data(), nullptr, nullptr, n);
+}
+
+extern "C" int grains_get_transforms(grains_ctx_t* ctx, double* t, uint64_t n) {
+  if (!ctx || !t) return GRAINS_EINVAL;
+  std::vector<double> pos(3 * n), q(4 * n);
+  int rc = grains_get_state(ctx, pos.data(), q.data(), nullptr, nullptr, n);
+  if (rc) return rc;
+  for (uint64_t i = 0; i < n; ++i) {
+    M3<double> R = rot(Qt<double>{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]});
+    std::copy(R.m, R.m + 9, t + 12 * i);
+    t[12 * i + 9] = pos[3 * i], t[12 * i + 10] = pos[3 * i + 1], t[12 * i + 11] = pos[3 * i + 2];
+  }
+  return GRAINS_OK;
+}
+
+// one step, enqueued on ctx->stream (also what a graph captures)
+static void enqueue_step(grains_ctx* ctx) {
+  DevScene& d = ctx->d;
+  hipStream_t s = ctx->stream;
+  if (d.precision == GRAINS_FP32_MIXED) launch_fill_u32(d.refine_count, 1, 0u, s);
+  launch_hash(d, s);
+  launch_sort(d, ctx->passes, &ctx->keys_s, &ctx->idx_s, s);
+  launch_cells(d, ctx->keys_s, ctx->idx_s, s);
+  launch_candidates(d, ctx->keys_s, ctx->idx_s, s);
+  launch_scan(d.cand_count, d.entry_off, d.n, d.scan_blocks, s);
+  launch_expand(d, ctx->idx_s, s);
+  if (d.precision == GRAINS_FP64) {
+    launch_narrow64(d, s);
+    launch_integrate64(d, s);
+  } else {
+    launch_narrow32(d, s);
+    launch_refine64(d, s);
+    launch_integrate32(d, s);
+  }
+}
+
+static int read_stats(grains_ctx* ctx, uint32_t steps) {
+  unsigned long long c[kNumCounters];
+  GR_HIP(hipMemcpy(c, ctx->d.counters, sizeof(c), hipMemcpyDeviceToHost));
+  grains_stats_t& st = ctx->stats;
+  st.steps = steps;
+  st.candidate_pairs = c[kCntCandidates];
+  st.gjk_calls = c[kCntGjk];
+  st.gjk_iterations = c[kCntGjkIters];
+  st.contacts = c[kCntContacts];
+  st.wall_contacts = c[kCntWall];
+  st.out_of_domain = c[kCntOod];
+  st.deep_contacts = c[kCntDeep];
+  st.fp64_refines = c[kCntRefine];
+  st.neighbor_overflow = c[kCntOverflow];
+  st.max_candidates = (uint32_t)c[kCntMaxCand];
+  st.sort_passes = (uint32_t)ctx->passes;
+  return GRAINS_OK;
+}
+
+static int run_steps(grains_ctx* ctx, uint32_t n_steps, bool graph) {
+  int rc = finalize(ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  launch_fill_u32((uint32_t*)ctx->d.counters, 2 * kNumCounters, 0u, s);
+  if (graph && !ctx->graph_exec) {
+    GR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue_step(ctx);
+    GR_HIP(hipStreamEndCapture(s, &ctx->graph));
+    GR_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+  }
+  GR_HIP(hipEventRecord(ctx->ev0, s));
+  for (uint32_t k = 0; k < n_steps; ++k) {
+    if (graph)
+      GR_HIP(hipGraphLaunch(ctx->graph_exec, s));
+    else
+      enqueue_step(ctx);
+  }
+  GR_HIP(hipGetLastError());
+  GR_HIP(hipEventRecord(ctx->ev1, s));
+  GR_HIP(hipStreamSynchronize(s));
+  float ms = 0;
+  GR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  rc = read_stats(ctx, n_steps);
+  ctx->stats.elapsed_ms = ms;
+  if (rc) return rc;
+  if (ctx->stats.neighbor_overflow)
+    return fail(ctx, GRAINS_EOVERFLOW, "candidate capacity max_neighbors exceeded (" +
+                                           std::to_string(ctx->stats.max_candidates) + " needed)");
+  return GRAINS_OK;
+}
+
+extern "C" int grains_step(grains_ctx_t* ctx, uint32_t n_steps) {
+  if (!ctx) return GRAINS_EINVAL;
+  return run_steps(ctx, n_steps, false);
+}
+
+extern "C" int grains_step_graph(grains_ctx_t* ctx, uint32_t n_steps) {
+  if (!ctx) return GRAINS_EINVAL;
+  return run_steps(ctx, n_steps, true);
+}
+
+extern "C" int grains_get_cell_keys(grains_ctx_t* ctx, uint32_t* keys, uint32_t* sorted, uint64_t n) {
+  if (!ctx) return GRAINS_EINVAL;
+  if (n != ctx->n || !ctx->layout_ok || !ctx->idx_s) return fail(ctx, GRAINS_ESTATE, "no step run yet");
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  if (keys) GR_HIP(hipMemcpy(keys, ctx->d.key, n * 4, hipMemcpyDeviceToHost));
+  if (sorted) GR_HIP(hipMemcpy(sorted, ctx->idx_s, n * 4, hipMemcpyDeviceToHost));
+  return GRAINS_OK;
+}
+
+extern "C" int grains_get_pairs(grains_ctx_t* ctx, int which, uint32_t* ij, uint64_t cap, uint64_t* m) {
+  if (!ctx || !m) return GRAINS_EINVAL;
+  if (!ctx->layout_ok || !ctx->idx_s) return fail(ctx, GRAINS_ESTATE, "no step run yet");
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  const uint64_t n = ctx->n;
+  std::vector<uint32_t> off(n + 1), rank(n);
+  GR_HIP(hipMemcpy(off.data(), ctx->d.entry_off, (n + 1) * 4, hipMemcpyDeviceToHost));
+  GR_HIP(hipMemcpy(rank.data(), ctx->d.rank, n * 4, hipMemcpyDeviceToHost));
+  const uint64_t E = off[n];
+  std::vector<uint32_t> pj(E), fl(E);
+  if (E) {
+    GR_HIP(hipMemcpy(pj.data(), ctx->d.pair_j, E * 4, hipMemcpyDeviceToHost));
+    GR_HIP(hipMemcpy(fl.data(), ctx->d.pair_flag, E * 4, hipMemcpyDeviceToHost));
+  }
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t s = rank[i];
+    for (uint32_t e = off[s]; e < off[s + 1]; ++e) {
+      if (which == GRAINS_PAIRS_CONTACT && !(fl[e] & kFlagContact)) continue;
+      if (ij && k < cap) {
+        ij[2 * k] = (uint32_t)i;
+        ij[2 * k + 1] = pj[e];
+      }
+      ++k;
+    }
+  }
+  *m = k;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_get_contact_count(grains_ctx_t* ctx, int32_t* counts, uint64_t n) {
+  if (!ctx || !counts) return GRAINS_EINVAL;
+  if (n != ctx->n) return fail(ctx, GRAINS_EINVAL, "particle count mismatch");
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  GR_HIP(hipMemcpy(counts, ctx->d.contact_count, n * 4, hipMemcpyDeviceToHost));
+  return GRAINS_OK;
+}
+
+extern "C" int grains_reset_contact_count(grains_ctx_t* ctx) {
+  if (!ctx || !ctx->n) return GRAINS_EINVAL;
+  launch_fill_u32((uint32_t*)ctx->d.contact_count, ctx->n, 0u, ctx->stream);
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  return GRAINS_OK;
+}
+
+extern "C" int grains_get_torce(grains_ctx_t* ctx, double* force, double* torque, uint64_t n) {
+  if (!ctx) return GRAINS_EINVAL;
+  if (n != ctx->n) return fail(ctx, GRAINS_EINVAL, "particle count mismatch");
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<double> buf(3 * n);
+  for (int w = 0; w < 2; ++w) {
+    double* dst = w ? torque : force;
+    if (!dst) continue;
+    GR_HIP(hipMemcpy(buf.data(), w ? ctx->d.last_torque : ctx->d.last_force, 3 * n * 8, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) dst[3 * i + k] = buf[k * n + i];
+  }
+  return GRAINS_OK;
+}
+
+extern "C" int grains_get_stats(grains_ctx_t* ctx, grains_stats_t* st) {
+  if (!ctx || !st) return GRAINS_EINVAL;
+  *st = ctx->stats;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_gjk_batch_device(grains_ctx_t* ctx, const uint32_t* sa, const uint32_t* sb, const double* ta,
+                                       const double* tb, uint64_t n, double* dist, int32_t* iters, float* ms) {
+  if (!ctx || !sa || !sb || !ta || !tb || !dist) return GRAINS_EINVAL;
+  if (ctx->shapes.empty()) return fail(ctx, GRAINS_ESTATE, "no shapes");
+  // shape tables only (no particles needed)
+  const size_t S = ctx->shapes.size();
+  const bool quirks = (ctx->cfg.flags & GRAINS_REF_QUIRKS) != 0;
+  std::vector<ShapeNP<double>> np64(S);
+  std::vector<ShapeNP<float>> np32(S);
+  std::vector<double> v64;
+  for (size_t i = 0; i < S; ++i) {
+    int32_t voff = (int32_t)(v64.size() / 3);
+    np64[i] = to_np<double>(ctx->shapes[i], quirks, voff);
+    np32[i] = to_np<float>(ctx->shapes[i], quirks, voff);
+    v64.insert(v64.end(), ctx->shapes[i].verts.begin(), ctx->shapes[i].verts.end());
+  }
+  if (v64.empty()) v64.assign(3, 0.0);
+  std::vector<float> v32(v64.begin(), v64.end());
+  std::vector<void*> tmp;
+  ShapeNP<double>* p64;
+  ShapeNP<float>* p32;
+  double* pv64;
+  float* pv32;
+  int rc;
+  if ((rc = dalloc(ctx, &p64, S * sizeof(ShapeNP<double>), tmp)) || (rc = dalloc(ctx, &p32, S * sizeof(ShapeNP<float>), tmp)) ||
+      (rc = dalloc(ctx, &pv64, v64.size() * 8, tmp)) || (rc = dalloc(ctx, &pv32, v32.size() * 4, tmp))) {
+    free_list(tmp);
+    return rc;
+  }
+  hipStream_t s = ctx->stream;
+  (void)hipMemcpy(p64, np64.data(), S * sizeof(ShapeNP<double>), hipMemcpyHostToDevice);
+  (void)hipMemcpy(p32, np32.data(), S * sizeof(ShapeNP<float>), hipMemcpyHostToDevice);
+  (void)hipMemcpy(pv64, v64.data(), v64.size() * 8, hipMemcpyHostToDevice);
+  (void)hipMemcpy(pv32, v32.data(), v32.size() * 4, hipMemcpyHostToDevice);
+  const GjkArgs args = {ctx->cfg.flags, ctx->cfg.gjk_rel_tol};
+  (void)hipEventRecord(ctx->ev0, s);
+  if (ctx->cfg.precision == GRAINS_FP64)
+    launch_gjk_batch64(p64, pv64, sa, sb, ta, tb, n, args, dist, nullptr, nullptr, iters, s);
+  else
+    launch_gjk_batch32(p32, pv32, sa, sb, ta, tb, n, args, dist, nullptr, nullptr, iters, s);
+  (void)hipEventRecord(ctx->ev1, s);
+  hipError_t e = hipStreamSynchronize(s);
+  float t = 0;
+  (void)hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+  if (ms) *ms = t;
+  free_list(tmp);
+  if (e != hipSuccess) return fail(ctx, GRAINS_EDEVICE, hipGetErrorString(e));
+  return GRAINS_OK;
+}
+
+extern "C" int grains_gjk_batch(grains_ctx_t* ctx, const uint32_t* sa, const uint32_t* sb, const double* ta,
+                                const double* tb, uint64_t n, double* dist, double* pa, double* pb, int32_t* iters) {
+  if (!ctx || !sa || !sb || !ta || !tb || !dist || !n) return GRAINS_EINVAL;
+  for (uint64_t k = 0; k < n; ++k)
+    if (sa[k] >= ctx->shapes.size() || sb[k] >= ctx->shapes.size())
+      return fail(ctx, GRAINS_EINVAL, "shape index out of range");
+  std::vector<void*> tmp;
+  uint32_t *dsa, *dsb;
+  double *dta, *dtb, *dd, *dpa, *dpb;
+  int32_t* dit;
+  int rc;
+  if ((rc = dalloc(ctx, &dsa, n * 4, tmp)) || (rc = dalloc(ctx, &dsb, n * 4, tmp)) ||
+      (rc = dalloc(ctx, &dta, n * 96, tmp)) || (rc = dalloc(ctx, &dtb, n * 96, tmp)) ||
+      (rc = dalloc(ctx, &dd, n * 8, tmp)) || (rc = dalloc(ctx, &dpa, n * 24, tmp)) ||
+      (rc = dalloc(ctx, &dpb, n * 24, tmp)) || (rc = dalloc(ctx, &dit, n * 4, tmp))) {
+    free_list(tmp);
+    return rc;
+  }
+  (void)hipMemcpy(dsa, sa, n * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dsb, sb, n * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dta, ta, n * 96, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dtb, tb, n * 96, hipMemcpyHostToDevice);
+  // shape tables
+  const size_t S = ctx->shapes.size();
+  const bool quirks = (ctx->cfg.flags & GRAINS_REF_QUIRKS) != 0;
+  std::vector<ShapeNP<double>> np64(S);
+  std::vector<ShapeNP<float>> np32(S);
+  std::vector<double> v64;
+  for (size_t i = 0; i < S; ++i) {
+    int32_t voff = (int32_t)(v64.size() / 3);
+    np64[i] = to_np<double>(ctx->shapes[i], quirks, voff);
+    np32[i] = to_np<float>(ctx->shapes[i], quirks, voff);
+    v64.insert(v64.end(), ctx->shapes[i].verts.begin(), ctx->shapes[i].verts.end());
+  }
+  if (v64.empty()) v64.assign(3, 0.0);
+  std::vector<float> v32(v64.begin(), v64.end());
+  ShapeNP<double>* p64;
+  ShapeNP<float>* p32;
+  double* pv64;
+  float* pv32;
+  if ((rc = dalloc(ctx, &p64, S * sizeof(ShapeNP<double>), tmp)) || (rc = dalloc(ctx, &p32, S * sizeof(ShapeNP<float>), tmp)) ||
+      (rc = dalloc(ctx, &pv64, v64.size() * 8, tmp)) || (rc = dalloc(ctx, &pv32, v32.size() * 4, tmp))) {
+    free_list(tmp);
+    return rc;
+  }
+  (void)hipMemcpy(p64, np64.data(), S * sizeof(ShapeNP<double>), hipMemcpyHostToDevice);
+  (void)hipMemcpy(p32, np32.data(), S * sizeof(ShapeNP<float>), hipMemcpyHostToDevice);
+  (void)hipMemcpy(pv64, v64.data(), v64.size() * 8, hipMemcpyHostToDevice);
+  (void)hipMemcpy(pv32, v32.data(), v32.size() * 4, hipMemcpyHostToDevice);
+  const GjkArgs args = {ctx->cfg.flags, ctx->cfg.gjk_rel_tol};
+  if (ctx->cfg.precision == GRAINS_FP64)
+    launch_gjk_batch64(p64, pv64, dsa, dsb, dta, dtb, n, args, dd, dpa, dpb, dit, ctx->stream);
+  else
+    launch_gjk_batch32(p32, pv32, dsa, dsb, dta, dtb, n, args, dd, dpa, dpb, dit, ctx->stream);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) {
+    (void)hipMemcpy(dist, dd, n * 8, hipMemcpyDeviceToHost);
+    if (pa) (void)hipMemcpy(pa, dpa, n * 24, hipMemcpyDeviceToHost);
+    if (pb) (void)hipMemcpy(pb, dpb, n * 24, hipMemcpyDeviceToHost);
+    if (iters) (void)hipMemcpy(iters, dit, n * 4, hipMemcpyDeviceToHost);
+  }
+  free_list(tmp);
+  if (e != hipSuccess) return fail(ctx, GRAINS_EDEVICE, hipGetErrorString(e));
+  return GRAINS_OK;
+}
+
+// snapshot layout: u64 n, then pos[3n], quat[4n], vel[3n], omega[3n] (FP64,
+// component planes), contact counts[n] (i32)
+extern "C" int grains_snapshot(grains_ctx_t* ctx, void* bytes, size_t* size) {
+  if (!ctx || !size) return GRAINS_EINVAL;
+  const uint64_t n = ctx->n;
+  const size_t need = 8 + 13 * n * 8 + n * 4;
+  if (!bytes) {
+    *size = need;
+    return GRAINS_OK;
+  }
+  if (*size < need) return fail(ctx, GRAINS_EINVAL, "snapshot buffer too small");
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  char* p = (char*)bytes;
+  std::memcpy(p, &n, 8);
+  p += 8;
+  GR_HIP(hipMemcpy(p, ctx->d.pos, 3 * n * 8, hipMemcpyDeviceToHost));
+  p += 3 * n * 8;
+  GR_HIP(hipMemcpy(p, ctx->d.quat, 4 * n * 8, hipMemcpyDeviceToHost));
+  p += 4 * n * 8;
+  GR_HIP(hipMemcpy(p, ctx->d.vel, 3 * n * 8, hipMemcpyDeviceToHost));
+  p += 3 * n * 8;
+  GR_HIP(hipMemcpy(p, ctx->d.omega, 3 * n * 8, hipMemcpyDeviceToHost));
+  p += 3 * n * 8;
+  GR_HIP(hipMemcpy(p, ctx->d.contact_count, n * 4, hipMemcpyDeviceToHost));
+  *size = need;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_restore(grains_ctx_t* ctx, const void* bytes, size_t size) {
+  if (!ctx || !bytes) return GRAINS_EINVAL;
+  const uint64_t n = ctx->n;
+  uint64_t n0;
+  if (size < 8) return GRAINS_EINVAL;
+  std::memcpy(&n0, bytes, 8);
+  if (n0 != n || size < 8 + 13 * n * 8 + n * 4) return fail(ctx, GRAINS_EINVAL, "snapshot does not match");
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  const char* p = (const char*)bytes + 8;
+  GR_HIP(hipMemcpy(ctx->d.pos, p, 3 * n * 8, hipMemcpyHostToDevice));
+  p += 3 * n * 8;
+  GR_HIP(hipMemcpy(ctx->d.quat, p, 4 * n * 8, hipMemcpyHostToDevice));
+  p += 4 * n * 8;
+  GR_HIP(hipMemcpy(ctx->d.vel, p, 3 * n * 8, hipMemcpyHostToDevice));
+  p += 3 * n * 8;
+  GR_HIP(hipMemcpy(ctx->d.omega, p, 3 * n * 8, hipMemcpyHostToDevice));
+  p += 3 * n * 8;
+  GR_HIP(hipMemcpy(ctx->d.contact_count, p, n * 4, hipMemcpyHostToDevice));
+  return GRAINS_OK;
+}

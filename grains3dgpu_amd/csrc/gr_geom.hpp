@@ -1,0 +1,643 @@
+// gr_geom.hpp -- narrow phase of the DEM step on the device: support maps,
+// oriented-bounding-box test, GJK distance with Johnson's sub-algorithm,
+// contact dispatch and the Hooke spring-dashpot / Coulomb force.
+//
+// Everything is templated on the arithmetic type T (double for the plain
+// path, float for the FP32-mixed path).  The Johnson sub-algorithm keeps its
+// determinant cache det[16][4] and dot products dp[4][4] in registers: every
+// index is made a compile-time constant (the free slot `last` is dispatched
+// to one of four instantiations, subsets are enumerated with fold
+// expressions), so no private-memory scratch is needed per lane.
+#pragma once
+
+#include <utility>
+
+#include "../../include/grains_hip.h"
+#include "gr_math.hpp"
+
+#define GR_D __device__ __forceinline__
+
+namespace gr {
+
+// Narrow-phase view of one shape (per particle or obstacle).
+template <class T>
+struct alignas(16) ShapeNP {
+  int32_t kind, material, nv, voff;
+  T dims[5];   // full shape
+  T core[5];   // crust core (physics semantics)
+  T crust;
+  T rc;        // circumscribed radius used by the bounding-sphere test
+  T bbox[3];   // OBB half extents (BoundingBox of the full shape)
+  T mass;
+  T pad[2];
+};
+
+// Hooke parameters with the precomputed m_muen (HookeContactForceModel.cpp:56).
+template <class T>
+struct HookeP {
+  T kn, en, etat, muc, kr, muen;
+};
+
+struct GjkArgs {
+  uint32_t flags;
+  double rel;   // relative tolerance of the optimality gap
+};
+
+// ---- support maps -----------------------------------------------------------
+// sgn (MiscMath.hh:68-72)
+template <class T> GR_D T sgn(T x) { return T((T(0) < x) - (x < T(0))); }
+
+// Superquadric::support (Superquadric.cpp:169-205), the reference's pow chain.
+template <class T>
+GR_D V3<T> sq_support_pow(const T* d, V3<T> v) {
+  const T a = d[0], b = d[1], c = d[2], n1 = d[3], n2 = d[4];
+  const T ax = fabs(v.x), ay = fabs(v.y), az = fabs(v.z);
+  const T sx = sgn(v.x), sy = sgn(v.y), sz = sgn(v.z);
+  if (ax == T(0)) {
+    if (ay == T(0)) return {T(0), T(0), sz * c};
+    T alpha = pow(c / b * az / ay, T(1) / (n1 - T(1)));
+    T yt = T(1) / pow(T(1) + pow(alpha, n1), T(1) / n1);
+    return {T(0), sy * b * yt, sz * alpha * c * yt};
+  }
+  T alpha = pow(b / a * ay / ax, T(1) / (n2 - T(1)));
+  T temp = T(1) + pow(alpha, n2);
+  T gamma = pow(temp, (n1 - n2) / (n2 * (n1 - T(1))));
+  T beta = gamma * pow(c / a * az / ax, T(1) / (n1 - T(1)));
+  T xt = T(1) / pow(pow(temp, n1 / n2) + pow(beta, n1), T(1) / n1);
+  return {sx * a * xt, sy * alpha * b * xt, sz * beta * c * xt};
+}
+
+// native v_exp_f32 / v_log_f32 (base 2, about 1 ulp)
+GR_D float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+GR_D float flog2(float x) { return __builtin_amdgcn_logf(x); }
+
+// Per-shape constants of the FP32 log-domain superquadric support.
+struct SqLog {
+  float la, lb, lc;        // log2 of the semi-axes
+  float n1, n2, k1, k2, g, r, in1;
+};
+GR_D SqLog sq_log_prepare(const float* d) {
+  SqLog s;
+  s.la = flog2(d[0]);
+  s.lb = flog2(d[1]);
+  s.lc = flog2(d[2]);
+  s.n1 = d[3];
+  s.n2 = d[4];
+  s.k1 = 1.0f / (d[4] - 1.0f);                       // 1/(n2-1)
+  s.k2 = 1.0f / (d[3] - 1.0f);                       // 1/(n1-1)
+  s.g = (d[3] - d[4]) / (d[4] * (d[3] - 1.0f));     // (n1-n2)/(n2(n1-1))
+  s.r = d[3] / d[4];                                 // n1/n2
+  s.in1 = 1.0f / d[3];
+  return s;
+}
+// log2(1 + 2^e), overflow-free
+GR_D float softplus2(float e) {
+  return e > 0.0f ? e + flog2(1.0f + fexp2(-e)) : flog2(1.0f + fexp2(e));
+}
+// log2(2^a + 2^b)
+GR_D float logadd2(float a, float b) {
+  float m = fmaxf(a, b);
+  return m + flog2(1.0f + fexp2(-fabsf(a - b)));
+}
+// Same closed form as sq_support_pow, evaluated in the log2 domain with the
+// native v_log_f32 / v_exp_f32: 10 transcendentals instead of 7 pow, and no
+// overflow of alpha^n2 for near-axis directions.
+GR_D V3<float> sq_support_log(const SqLog& s, V3<float> v) {
+  const float ax = fabsf(v.x), ay = fabsf(v.y), az = fabsf(v.z);
+  const float sx = sgn(v.x), sy = sgn(v.y), sz = sgn(v.z);
+  if (ax == 0.0f) {
+    if (ay == 0.0f) return {0.0f, 0.0f, sz * fexp2(s.lc)};
+    float La = (s.lc - s.lb + flog2(az) - flog2(ay)) * s.k2;
+    float Lyt = -s.in1 * softplus2(s.n1 * La);
+    return {0.0f, sy * fexp2(s.lb + Lyt), sz * fexp2(s.lc + La + Lyt)};
+  }
+  const float lx = flog2(ax);
+  float La = (s.lb - s.la + flog2(ay) - lx) * s.k1;
+  float Lt = softplus2(s.n2 * La);
+  float Lb = Lt * s.g + (s.lc - s.la + flog2(az) - lx) * s.k2;
+  float Lxt = -s.in1 * logadd2(Lt * s.r, s.n1 * Lb);
+  return {sx * fexp2(s.la + Lxt), sy * fexp2(s.lb + La + Lxt), sz * fexp2(s.lc + Lb + Lxt)};
+}
+
+// Per-lane shape context for GJK: which dims, and FP32 log constants.
+template <class T>
+struct SupCtx {
+  int kind, nv;
+  const T* d;           // dims in use (full or core)
+  const T* verts;       // polyhedron vertices in use (x,y,z triplets)
+  SqLog lg;             // used when T = float and kind = SUPERQUADRIC
+};
+
+template <class T>
+GR_D SupCtx<T> sup_ctx(const ShapeNP<T>& s, bool core, const T* verts_full, const T* verts_core) {
+  SupCtx<T> c;
+  c.kind = s.kind;
+  c.nv = s.nv;
+  c.d = core ? s.core : s.dims;
+  c.verts = (core ? verts_core : verts_full) + 3 * (size_t)s.voff;
+  if constexpr (sizeof(T) == 4) {
+    if (s.kind == GRAINS_SUPERQUADRIC) c.lg = sq_log_prepare(c.d);
+  }
+  return c;
+}
+
+// Sphere.cpp:111-114, Box.cpp:129-134, Rectangle.cpp:115-120,
+// Superquadric.cpp:169-205; polyhedron: first vertex maximising v.x.
+template <class T>
+GR_D V3<T> support(const SupCtx<T>& c, V3<T> v) {
+  const T* d = c.d;
+  switch (c.kind) {
+    case GRAINS_SPHERE: {
+      T n = norm(v);
+      return d[0] * (v / n);
+    }
+    case GRAINS_BOX:
+      return {v.x < T(0) ? -d[0] : d[0], v.y < T(0) ? -d[1] : d[1], v.z < T(0) ? -d[2] : d[2]};
+    case GRAINS_SUPERQUADRIC:
+      if constexpr (sizeof(T) == 4)
+        return sq_support_log(c.lg, v);
+      else
+        return sq_support_pow(d, v);
+    case GRAINS_RECTANGLE:
+      return {v.x < T(0) ? -d[0] : d[0], v.y < T(0) ? -d[1] : d[1], T(0)};
+    case GRAINS_POLYHEDRON: {
+      const T* P = c.verts;
+      V3<T> best = {P[0], P[1], P[2]};
+      T bd = dot(v, best);
+      for (int i = 1; i < c.nv; ++i) {
+        V3<T> p = {P[3 * i], P[3 * i + 1], P[3 * i + 2]};
+        T dd = dot(v, p);
+        if (dd > bd) {
+          bd = dd;
+          best = p;
+        }
+      }
+      return best;
+    }
+  }
+  return {T(0), T(0), T(0)};
+}
+
+// ---- OBB, Grains/CollisionDetection/src/OBB.cpp:32-94 -----------------------
+template <class T>
+GR_D bool obb_intersect(const T* a, const T* b, const Tr<T>& A, const Tr<T>& B) {
+  const T LO = Eps<T>::LOW;
+  M3<T> ori = transpose(A.R);
+  V3<T> c3 = mv(ori, B.o - A.o);
+  ori = mmul(ori, B.R);
+  const T cen[3] = {c3.x, c3.y, c3.z};
+  T o[3][3], oa[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      o[i][j] = ori.m[3 * i + j];
+      oa[i][j] = fabs(o[i][j]) + LO;
+    }
+  bool sep = false;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    sep |= fabs(cen[i]) > (a[i] + b[0] * oa[i][0] + b[1] * oa[i][1] + b[2] * oa[i][2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    sep |= fabs(cen[0] * o[0][i] + cen[1] * o[1][i] + cen[2] * o[2][i]) >
+           (b[i] + a[0] * oa[0][i] + a[1] * oa[1][i] + a[2] * oa[2][i]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      sep |= fabs(cen[i2] * o[i1][j] - cen[i1] * o[i2][j]) >
+             (a[i1] * oa[i2][j] + a[i2] * oa[i1][j] + b[j1] * oa[i][j2] + b[j2] * oa[i][j1]);
+    }
+  return !sep;
+}
+
+// ---- Johnson's distance sub-algorithm (GJK_JH.cpp:9-207) ----------------------
+template <class T>
+struct JH {
+  V3<T> y[4], p[4], q[4];
+  T det[16][4];
+  T dp[4][4];
+};
+
+template <class T, int S>
+GR_D bool jh_valid(const JH<T>& st, unsigned all_bits) {
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const unsigned bit = 1u << i;
+    if (all_bits & bit) {
+      if (S & bit) {
+        if (st.det[S][i] <= Eps<T>::MID) ok = false;
+      } else if (st.det[S | bit][i] > T(0)) {
+        ok = false;
+      }
+    }
+  }
+  return ok;
+}
+
+template <class T, int S>
+GR_D bool jh_proper(const JH<T>& st) {
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if ((S & (1 << i)) && st.det[S][i] <= Eps<T>::MID) ok = false;
+  return ok;
+}
+
+template <class T, int S>
+GR_D V3<T> jh_vector(const JH<T>& st) {
+  T sum = T(0);
+  V3<T> v = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (S & (1 << i)) {
+      sum += st.det[S][i];
+      v = v + st.det[S][i] * st.y[i];
+    }
+  return (T(1) / sum) * v;
+}
+
+template <class T, int S>
+GR_D void jh_points(const JH<T>& st, V3<T>& p1, V3<T>& p2) {
+  T sum = T(0);
+  p1 = {T(0), T(0), T(0)};
+  p2 = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (S & (1 << i)) {
+      sum += st.det[S][i];
+      p1 = p1 + st.det[S][i] * st.p[i];
+      p2 = p2 + st.det[S][i] * st.q[i];
+    }
+  T s = T(1) / sum;
+  p1 = s * p1;
+  p2 = s * p2;
+}
+
+template <class T, int... Is>
+GR_D void jh_points_dispatch(const JH<T>& st, unsigned bits, V3<T>& p1, V3<T>& p2,
+                             std::integer_sequence<int, Is...>) {
+  ((bits == unsigned(Is + 1) ? jh_points<T, Is + 1>(st, p1, p2) : void()), ...);
+}
+
+// one subset S of `bits` tried in the descending scan of closest()
+template <class T, int L, int S>
+GR_D void jh_try(const JH<T>& st, unsigned bits, unsigned all_bits, bool& found, unsigned& nb, V3<T>& v) {
+  if constexpr ((S & (1 << L)) == 0) {
+    if (!found && (unsigned(S) & bits) == unsigned(S)) {
+      if (jh_valid<T, S | (1 << L)>(st, all_bits)) {
+        nb = S | (1 << L);
+        v = jh_vector<T, S | (1 << L)>(st);
+        found = true;
+      }
+    }
+  }
+}
+template <class T, int L, int... Is>
+GR_D void jh_try_all(const JH<T>& st, unsigned bits, unsigned all_bits, bool& found, unsigned& nb,
+                     V3<T>& v, std::integer_sequence<int, Is...>) {
+  (jh_try<T, L, 15 - Is>(st, bits, all_bits, found, nb, v), ...);
+}
+// backup procedure: best proper subset of all_bits (descending scan)
+template <class T, int S>
+GR_D void jh_backup1(const JH<T>& st, unsigned all_bits, T& best, unsigned& nb, V3<T>& v) {
+  if ((unsigned(S) & all_bits) == unsigned(S)) {
+    if (jh_proper<T, S>(st)) {
+      V3<T> u = jh_vector<T, S>(st);
+      T d2 = norm2(u);
+      if (d2 < best) {
+        best = d2;
+        nb = S;
+        v = u;
+      }
+    }
+  }
+}
+template <class T, int... Is>
+GR_D void jh_backup_all(const JH<T>& st, unsigned all_bits, T& best, unsigned& nb, V3<T>& v,
+                        std::integer_sequence<int, Is...>) {
+  (jh_backup1<T, 15 - Is>(st, all_bits, best, nb, v), ...);
+}
+
+// computeDet + closest for the new point in slot L (GJK_JH.cpp:9-63, 143-194)
+template <class T, int L>
+GR_D bool jh_closest(JH<T>& S, unsigned& bits, unsigned all_bits, V3<T>& v) {
+  constexpr unsigned LB = 1u << L;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i != L && (bits & (1u << i))) {
+      T d = dot(S.y[i], S.y[L]);
+      S.dp[i][L] = d;
+      S.dp[L][i] = d;
+    }
+  S.dp[L][L] = dot(S.y[L], S.y[L]);
+  S.det[LB][L] = T(1);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j == L) continue;
+    const unsigned sj = 1u << j;
+    if (bits & sj) {
+      const unsigned s2 = sj | LB;
+      S.det[s2][j] = S.dp[L][L] - S.dp[L][j];
+      S.det[s2][L] = S.dp[j][j] - S.dp[j][L];
+#pragma unroll
+      for (int k = 0; k < j; ++k) {
+        if (k == L) continue;
+        const unsigned sk = 1u << k;
+        if (bits & sk) {
+          const unsigned s3 = sk | s2;
+          S.det[s3][k] = S.det[s2][j] * (S.dp[j][j] - S.dp[j][k]) + S.det[s2][L] * (S.dp[L][j] - S.dp[L][k]);
+          S.det[s3][j] = S.det[sk | LB][k] * (S.dp[k][k] - S.dp[k][j]) +
+                         S.det[sk | LB][L] * (S.dp[L][k] - S.dp[L][j]);
+          S.det[s3][L] = S.det[sk | sj][k] * (S.dp[k][k] - S.dp[k][L]) +
+                         S.det[sk | sj][j] * (S.dp[j][k] - S.dp[j][L]);
+        }
+      }
+    }
+  }
+  if (all_bits == 15) {
+    auto& det = S.det;
+    auto& dp = S.dp;
+    det[15][0] = det[14][1] * (dp[1][1] - dp[1][0]) + det[14][2] * (dp[2][1] - dp[2][0]) +
+                 det[14][3] * (dp[3][1] - dp[3][0]);
+    det[15][1] = det[13][0] * (dp[0][0] - dp[0][1]) + det[13][2] * (dp[2][0] - dp[2][1]) +
+                 det[13][3] * (dp[3][0] - dp[3][1]);
+    det[15][2] = det[11][0] * (dp[0][0] - dp[0][2]) + det[11][1] * (dp[1][0] - dp[1][2]) +
+                 det[11][3] * (dp[3][0] - dp[3][2]);
+    det[15][3] = det[7][0] * (dp[0][0] - dp[0][3]) + det[7][1] * (dp[1][0] - dp[1][3]) +
+                 det[7][2] * (dp[2][0] - dp[2][3]);
+  }
+  bool found = false;
+  unsigned nb = bits;
+  jh_try_all<T, L>(S, bits, all_bits, found, nb, v, std::make_integer_sequence<int, 15>{});
+  if (found) {
+    bits = nb;
+    return true;
+  }
+  if (jh_valid<T, LB>(S, all_bits)) {
+    bits = LB;
+    v = S.y[L];
+    return true;
+  }
+  T best = T(INFINITY);
+  jh_backup_all<T>(S, all_bits, best, nb, v, std::make_integer_sequence<int, 15>{});
+  bits = nb;
+  return false;
+}
+
+// Result of one GJK distance query.
+template <class T>
+struct GjkOut {
+  T dist;
+  V3<T> pa, pb;   // witness points in the local frames of A and B
+  int iters;
+  bool deep;      // physics semantics: crust cores overlap
+  bool backup;    // the sub-algorithm fell back to its backup procedure
+};
+
+// computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391).  REF_QUIRKS: the
+// reference exactly, incl. v0 = s_A(0) - b2a(s_B(0)) and the break on
+// mu < 1e-4*rel.  Physics: v0 = centre difference, the standard
+// van den Bergen termination, |v| of the backup simplex, witnesses of the
+// last support pair when no simplex was built; NESTEROV adds momentum on the
+// support direction (Montaut et al. 2022) with the lower bound d.w/|d|.
+template <class T>
+GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
+                      const GjkArgs& args) {
+  const bool quirks = (args.flags & GRAINS_REF_QUIRKS) != 0;
+  const bool nesterov = !quirks && (args.flags & GRAINS_NESTEROV);
+  const T rel = args.rel > 0 ? T(args.rel) : Eps<T>::LOW;
+  const T abs_err = T(1.e-4) * rel;
+  JH<T> S;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S.det[i][k] = T(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S.dp[i][k] = T(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) S.y[i] = S.p[i] = S.q[i] = {T(0), T(0), T(0)};
+  unsigned bits = 0, all_bits = 0;
+  int last = 0;
+  T mu = T(0);
+  int num_iter = 0;
+  const Tr<T> b2a = relative(a2w, b2w);
+  const V3<T> zero = {T(0), T(0), T(0)};
+  V3<T> v = quirks ? support(A, zero) - apply(b2a, support(B, zero)) : -b2a.o;
+  V3<T> w = zero, pl = zero, ql = zero;
+  T dist = norm(v);
+  V3<T> dir = v, wprev = v;
+  bool accel = nesterov;
+  bool backup = false;
+  bool ran = false;
+  while (bits < 15 && dist > Eps<T>::HIGH && num_iter < 1000) {
+    last = 0;
+    unsigned last_bit = 1;
+    while (bits & last_bit) {
+      ++last;
+      last_bit <<= 1;
+    }
+    V3<T> d = v;
+    if (accel) {
+      T delta = T(num_iter + 1) / T(num_iter + 3);
+      V3<T> yk = delta * v + (T(1) - delta) * wprev;
+      T ny = norm(yk), nd = norm(dir);
+      if (num_iter != 0 && ny > Eps<T>::HIGH && nd > Eps<T>::HIGH)
+        d = (delta / nd) * dir + ((T(1) - delta) / ny) * yk;
+      dir = d;
+    }
+    pl = support(A, -d);
+    ql = support(B, vm(d, b2a.R));
+    w = pl - apply(b2a, ql);
+    ran = true;
+    T lb = accel ? dot(d, w) / norm(d) : dot(v, w) / dist;
+    if (mu < lb) mu = lb;
+    if (quirks) {
+      if (dist - mu <= dist * rel || mu < abs_err) break;
+    } else if (dist - mu <= dist * rel) {
+      break;
+    }
+    bool degen = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if ((all_bits & (1u << i)) && same(S.y[i], w)) degen = true;
+    if (degen) {
+      if (accel) {
+        accel = false;
+        continue;
+      }
+      break;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i == last) {
+        S.y[i] = w;
+        S.p[i] = pl;
+        S.q[i] = ql;
+      }
+    all_bits = bits | last_bit;
+    ++num_iter;
+    const T dist_before = dist;
+    bool ok;
+    switch (last) {
+      case 0: ok = jh_closest<T, 0>(S, bits, all_bits, v); break;
+      case 1: ok = jh_closest<T, 1>(S, bits, all_bits, v); break;
+      case 2: ok = jh_closest<T, 2>(S, bits, all_bits, v); break;
+      default: ok = jh_closest<T, 3>(S, bits, all_bits, v); break;
+    }
+    if (!ok) {
+      backup = true;
+      if (!quirks) dist = norm(v);
+      break;
+    }
+    dist = norm(v);
+    wprev = w;
+    if (accel && dist > dist_before * (T(1) - T(1e-12))) accel = false;
+  }
+  GjkOut<T> out;
+  if (!quirks && bits == 0) {
+    out.pa = ran ? pl : zero;
+    out.pb = ran ? ql : zero;
+  } else {
+    jh_points_dispatch<T>(S, bits, out.pa, out.pb, std::make_integer_sequence<int, 15>{});
+    if (bits == 0) {  // reference: computePoints with an empty simplex gives 0 * (1/0)
+      T s = T(1) / T(0);
+      out.pa = s * zero;
+      out.pb = s * zero;
+    }
+  }
+  out.dist = dist;
+  out.iters = num_iter;
+  out.deep = !quirks && (bits == 15 || dist <= Eps<T>::HIGH);
+  out.backup = backup;
+  return out;
+}
+
+// ---- contact information -------------------------------------------------------
+template <class T>
+struct CInfo {
+  V3<T> cp, cv;
+  T overlap;
+};
+template <class T> GR_D CInfo<T> no_contact() { return {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0)}; }
+
+// closestPointsSpheres (CollisionDetection.cpp:27-48)
+template <class T>
+GR_D CInfo<T> cp_spheres(T rA, T rB, const Tr<T>& a2w, const Tr<T>& b2w) {
+  V3<T> cenA = a2w.o;
+  V3<T> vecBA = b2w.o - cenA;
+  T overlap = norm(vecBA) - rA - rB;
+  vecBA = vecBA / norm(vecBA);
+  if (overlap < T(0)) return {cenA + (rA + T(0.5) * overlap) * vecBA, overlap * vecBA, overlap};
+  return no_contact<T>();
+}
+
+// closestPointsRectangle (CollisionDetection.cpp:54-90); A's support is the full shape
+template <class T>
+GR_D CInfo<T> cp_rectangle(const SupCtx<T>& A, const Tr<T>& a2w, const Tr<T>& b2w) {
+  V3<T> rPt = b2w.o;
+  V3<T> rNorm = {b2w.R.m[2], b2w.R.m[5], b2w.R.m[8]};
+  rNorm = copysign(T(1), dot(rNorm, a2w.o - rPt)) * rNorm;
+  V3<T> pointA = apply(a2w, support(A, vm(-rNorm, a2w.R)));
+  if (dot(rNorm, pointA - rPt) < T(0)) {
+    V3<T> pointB = dot(rPt - pointA, rNorm) * rNorm + pointA;
+    V3<T> cv = pointB - pointA;
+    return {T(0.5) * (pointA + pointB), cv, -norm(cv)};
+  }
+  return no_contact<T>();
+}
+
+struct NpStats {
+  int gjk;        // 1 when GJK ran
+  int iters;
+  int deep;
+  int backup;
+};
+
+// closestPointsRigidBodies (CollisionDetection.cpp:181-284).  Physics
+// semantics run GJK on the crust cores (the commented intent of lines 231-236).
+template <class T>
+GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<T>& a2w,
+                             const Tr<T>& b2w, const GjkArgs& args, const T* verts_full,
+                             const T* verts_core, NpStats& st) {
+  const bool quirks = (args.flags & GRAINS_REF_QUIRKS) != 0;
+  if (A.kind == GRAINS_SPHERE && B.kind == GRAINS_SPHERE) return cp_spheres(A.rc, B.rc, a2w, b2w);
+  if (B.kind == GRAINS_RECTANGLE) return cp_rectangle(sup_ctx(A, false, verts_full, verts_core), a2w, b2w);
+  T overlap = norm(b2w.o - a2w.o) - (A.rc + B.rc);
+  if (overlap < T(0)) {
+    if (obb_intersect(A.bbox, B.bbox, a2w, b2w)) {
+      SupCtx<T> ca = sup_ctx(A, !quirks, verts_full, verts_core);
+      SupCtx<T> cb = sup_ctx(B, !quirks, verts_full, verts_core);
+      GjkOut<T> g = gjk_jh(ca, cb, a2w, b2w, args);
+      st.gjk = 1;
+      st.iters = g.iters;
+      st.backup = g.backup;
+      T ctSum = A.crust + B.crust;
+      T distance = g.dist - ctSum;
+      if (g.deep) distance = -ctSum;
+      if (distance > T(0)) return no_contact<T>();
+      V3<T> ptA = apply(a2w, g.pa);
+      V3<T> ptB = apply(b2w, g.pb);
+      CInfo<T> c;
+      c.cp = (ptA + ptB) / T(2);
+      V3<T> cv = ptA - ptB;
+      if (g.deep) {
+        st.deep = 1;
+        cv = a2w.o - b2w.o;
+        c.cp = (a2w.o + b2w.o) / T(2);
+      }
+      cv = cv / norm(cv);
+      cv = round_tol(cv, Eps<T>::MID);
+      c.cv = (-distance) * cv;
+      c.overlap = distance;
+      return c;
+    }
+  }
+  return no_contact<T>();
+}
+
+// Hooke::performForcesCalculus + computeForces (HookeContactForceModel.cpp:91-186).
+// Returns the force on A; `lever` = contact point - A's origin.
+template <class T>
+GR_D void hooke(const HookeP<T>& h, const CInfo<T>& ci, V3<T> relVel, V3<T> relAngVel, T m1, T m2,
+                V3<T> lever, V3<T>& F, V3<T>& M) {
+  V3<T> pen = ci.cv;
+  V3<T> delFN = h.kn * pen;
+  pen = pen / norm(pen);
+  pen = round_tol(pen, Eps<T>::MID);
+  V3<T> v_n = dot(relVel, pen) * pen;
+  V3<T> v_t = relVel - v_n;
+  T normv_t = norm(v_t);
+  V3<T> tangent = {T(0), T(0), T(0)};
+  if (normv_t > Eps<T>::MID) tangent = v_t / normv_t;
+  T avmass = m1 * m2 / (m1 + m2);
+  T omega0 = sqrt(h.kn / avmass);
+  if (avmass == T(0)) {
+    avmass = m2 == T(0) ? T(0.5) * m1 : T(0.5) * m2;
+    omega0 = T(2) * sqrt(h.kn / avmass);
+  }
+  T muen = -omega0 * h.muen;
+  delFN = delFN + (-T(2) * muen * avmass) * v_n;
+  T normFN = norm(delFN);
+  V3<T> delFT = (-h.etat * T(2) * avmass) * v_t;
+  T fn = h.muc * normFN;
+  T ft = norm(delFT);
+  if (fn < ft) delFT = (-fn) * tangent;
+  F = delFN + delFT;
+  M = cross(lever, F);
+  if (h.kr != T(0)) {
+    V3<T> wn = dot(relAngVel, pen) * pen;
+    V3<T> wt = relAngVel - wn;
+    T normwt = norm(wt);
+    V3<T> delM = (-h.kr * normFN * T(0.001)) * wn;
+    if (normwt > Eps<T>::MID) delM = delM - (h.kr * normFN) * wt;
+    M = M + delM;
+  }
+}
+
+}  // namespace gr

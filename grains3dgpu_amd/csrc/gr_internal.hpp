@@ -1,0 +1,189 @@
+// gr_internal.hpp -- device data layout and launchers shared by the kernel
+// translation units (gr_broad.hip, gr_narrow64.hip, gr_narrow32.hip) and
+// the C-ABI (gr_capi.hip).
+//
+// HBM layout (N particles, E candidate entries, C cells):
+//   state    pos[3][N], quat[4][N] (x,y,z,w), vel[3][N], omega[3][N]   FP64 planes
+//   ids      shape[N], sim[N]                                          u32
+//   broad    key[N] (by particle), key/idx ping-pong [2][N], rank[N],
+//            cell_start[C], cell_end[C]                                u32
+//   pairs    cand[N][K], cand_count[N], entry_off[N+1],
+//            pair_i[E], pair_j[E], pair_flag[E]                        u32
+//            pair_res[E] = {F[3], M[3]} in the narrow-phase type
+//   shapes   ShapeNP<double>[S], ShapeNP<float>[S], ShapeDyn[S]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gr_geom.hpp"
+
+namespace gr {
+
+constexpr int kBlock = 256;          // threads per block of streaming kernels
+constexpr int kItems = 8;            // items per thread in sort / scan tiles
+constexpr int kTile = kBlock * kItems;
+constexpr double kPrefilterSlack = 1.e-10;
+
+// per-pair flag bits
+constexpr uint32_t kFlagContact = 1u;
+constexpr uint32_t kFlagGjk = 2u;
+constexpr uint32_t kFlagDeep = 4u;
+constexpr uint32_t kFlagRefine = 8u;
+
+// device counters (u64), summed over the steps of one grains_step call
+enum Counter {
+  kCntCandidates = 0,
+  kCntGjk,
+  kCntGjkIters,
+  kCntContacts,
+  kCntWall,
+  kCntOod,
+  kCntDeep,
+  kCntRefine,
+  kCntOverflow,
+  kCntMaxCand,   // stored as max
+  kCntBackup,
+  kNumCounters = 16
+};
+
+// One simulation's linked cell and forces.
+struct SimGrid {
+  double lo[3], hi[3], L[3];
+  double h;
+  double g[3];
+  uint32_t n3[3];
+  uint32_t base;         // first global cell
+  int32_t periodic;      // bitmask x=1, y=2, z=4
+  uint32_t table;        // first entry of its contact table block
+  uint32_t obs_begin, obs_end;  // its obstacles (obstacles sorted by sim)
+};
+
+// Dynamics of one shape (RigidBody mass properties, RigidBody.cpp:19-59).
+struct alignas(16) ShapeDyn {
+  double mass;
+  double inertia[6], inv[6];
+  double pad;
+};
+
+template <class T>
+struct alignas(16) PairRes {
+  T f[3];
+  T m[3];
+};
+
+struct DevScene {
+  uint64_t n;
+  uint32_t nsims, nobs, nmat;
+  // state (planes of length n)
+  double *pos, *quat, *vel, *omega;
+  uint32_t *shape, *sim;
+  // shapes
+  const ShapeNP<double>* np64;
+  const ShapeNP<float>* np32;
+  const ShapeDyn* dyn;
+  const double* rc_pf;          // prefilter radius per shape
+  const double* verts64;        // full polyhedron vertices
+  const double* cverts64;       // crust-core vertices
+  const float* verts32;
+  const float* cverts32;
+  const HookeP<double>* hooke64;
+  const HookeP<float>* hooke32;
+  const SimGrid* grids;
+  const uint32_t* obs_shape;    // sorted by sim
+  const double* obs_tr;         // 12 doubles each
+  // broad phase
+  uint32_t *key, *keyA, *idxA, *keyB, *idxB, *rank;
+  uint32_t *cell_start, *cell_end;
+  uint32_t ncells;
+  uint32_t *hist;               // 256 * n_tiles (+1)
+  uint32_t *scan_blocks;        // block sums of scans
+  // pairs
+  uint32_t K;
+  uint64_t ecap;
+  uint32_t *cand, *cand_count, *entry_off;
+  uint32_t *pair_i, *pair_j, *pair_flag;
+  void* pair_res;               // PairRes<T>[ecap]
+  uint32_t* refine_list;        // entries to re-run in FP64 (FP32-mixed)
+  uint32_t* refine_count;
+  // per particle outputs
+  int32_t* contact_count;
+  double *last_force, *last_torque;
+  unsigned long long* counters;
+  // config
+  double dt;
+  uint32_t flags;
+  double gjk_rel;
+  int precision;
+};
+
+// ---- launchers ------------------------------------------------------------------
+// gr_broad.hip
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+void launch_hash(const DevScene& d, hipStream_t s);
+// sorts (d.key, identity) into the returned buffers; passes = radix passes
+void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx_sorted, hipStream_t s);
+void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s);
+void launch_candidates(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s);
+void launch_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* block_sums, hipStream_t s);
+void launch_expand(const DevScene& d, const uint32_t* idx_s, hipStream_t s);
+int persistent_blocks();
+// gr_narrow64.hip / gr_narrow32.hip
+void launch_narrow64(const DevScene& d, hipStream_t s);
+void launch_narrow32(const DevScene& d, hipStream_t s);
+void launch_refine64(const DevScene& d, hipStream_t s);
+void launch_integrate64(const DevScene& d, hipStream_t s);
+void launch_integrate32(const DevScene& d, hipStream_t s);
+// standalone GJK distance batches (GrainsTestDev workload)
+void launch_gjk_batch64(const ShapeNP<double>* shapes, const double* verts, const uint32_t* sa,
+                        const uint32_t* sb, const double* ta, const double* tb, uint64_t n,
+                        GjkArgs args, double* dist, double* pa, double* pb, int32_t* iters,
+                        hipStream_t s);
+void launch_gjk_batch32(const ShapeNP<float>* shapes, const float* verts, const uint32_t* sa,
+                        const uint32_t* sb, const double* ta, const double* tb, uint64_t n,
+                        GjkArgs args, double* dist, double* pa, double* pb, int32_t* iters,
+                        hipStream_t s);
+
+// ---- small device helpers --------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
+  return v;
+}
+// add a per-lane count to a device counter, one atomic per wave
+__device__ __forceinline__ void count_add(unsigned long long* c, uint32_t v) {
+  uint32_t s = wave_sum(v);
+  if (__lane_id() == 0 && s) atomicAdd(c, (unsigned long long)s);
+}
+
+// Quaternion state and pose helpers
+__device__ __forceinline__ Qt<double> load_quat(const double* q, uint64_t n, uint64_t i) {
+  return {q[i], q[n + i], q[2 * n + i], q[3 * n + i]};
+}
+__device__ __forceinline__ V3<double> load3(const double* p, uint64_t n, uint64_t i) {
+  return {p[i], p[n + i], p[2 * n + i]};
+}
+__device__ __forceinline__ void store3(double* p, uint64_t n, uint64_t i, V3<double> v) {
+  p[i] = v.x;
+  p[n + i] = v.y;
+  p[2 * n + i] = v.z;
+}
+// minimum-image offset along periodic directions
+__device__ __forceinline__ V3<double> min_image(const SimGrid& g, V3<double> d) {
+  if (g.periodic & 1) d.x -= g.L[0] * rint(d.x / g.L[0]);
+  if (g.periodic & 2) d.y -= g.L[1] * rint(d.y / g.L[1]);
+  if (g.periodic & 4) d.z -= g.L[2] * rint(d.z / g.L[2]);
+  return d;
+}
+// ContactForceModelBuilderFactory::computeHash (ContactForceModelBuilderFactory.cpp:77-83)
+__device__ __forceinline__ uint32_t material_hash(uint32_t x, uint32_t y) {
+  uint32_t s = x < y ? x : y, l = x < y ? y : x;
+  return l * (l + 1) / 2 + s;
+}
+
+}  // namespace gr

@@ -1,0 +1,311 @@
+// gr_narrow.hpp -- per-entry narrow phase + contact force, per-particle
+// reduction + walls + gravity + explicit integration, and the standalone GJK
+// batch.  Included by gr_narrow64.hip (T = double, built without FMA
+// contraction) and gr_narrow32.hip (T = float, FP32-mixed).
+#pragma once
+
+#include "gr_internal.hpp"
+
+namespace gr {
+
+template <class T> __device__ __forceinline__ const ShapeNP<T>* np_table(const DevScene& d);
+template <> __device__ __forceinline__ const ShapeNP<double>* np_table<double>(const DevScene& d) { return d.np64; }
+template <> __device__ __forceinline__ const ShapeNP<float>* np_table<float>(const DevScene& d) { return d.np32; }
+template <class T> __device__ __forceinline__ const T* vt_full(const DevScene& d);
+template <> __device__ __forceinline__ const double* vt_full<double>(const DevScene& d) { return d.verts64; }
+template <> __device__ __forceinline__ const float* vt_full<float>(const DevScene& d) { return d.verts32; }
+template <class T> __device__ __forceinline__ const T* vt_core(const DevScene& d);
+template <> __device__ __forceinline__ const double* vt_core<double>(const DevScene& d) { return d.cverts64; }
+template <> __device__ __forceinline__ const float* vt_core<float>(const DevScene& d) { return d.cverts32; }
+template <class T> __device__ __forceinline__ const HookeP<T>* hk_table(const DevScene& d);
+template <> __device__ __forceinline__ const HookeP<double>* hk_table<double>(const DevScene& d) { return d.hooke64; }
+template <> __device__ __forceinline__ const HookeP<float>* hk_table<float>(const DevScene& d) { return d.hooke32; }
+
+// Poses of A and B for the narrow phase.  FP64: world coordinates, with B's
+// periodic image placed next to A exactly as the oracle does.  FP32: a frame
+// translated to A's centre, B's offset taken in FP64 before rounding.
+template <class T>
+__device__ __forceinline__ void pair_frames(const SimGrid& g, V3<double> xi, Qt<double> qi, V3<double> xj,
+                                            Qt<double> qj, Tr<T>& a2w, Tr<T>& b2w) {
+  V3<double> dd = xj - xi;
+  V3<double> dm = g.periodic ? min_image(g, dd) : dd;
+  if constexpr (sizeof(T) == 8) {
+    a2w.R = rot(qi);
+    a2w.o = xi;
+    b2w.R = rot(qj);
+    b2w.o = same(dm, dd) ? xj : xi + dm;
+  } else {
+    a2w.R = cvt<float>(rot(qi));
+    a2w.o = {0.0f, 0.0f, 0.0f};
+    b2w.R = cvt<float>(rot(qj));
+    b2w.o = cvt<float>(dm);
+  }
+}
+
+// Narrow phase + Hooke force of entry (i, j) from i's side
+// (ComponentManagerCPU.cpp:384-419).  Returns the pair flag.
+template <class T, class R>
+__device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t i, uint32_t j, PairRes<R>& out,
+                                               NpStats& st) {
+  const uint64_t n = d.n;
+  const ShapeNP<T>* np = np_table<T>(d);
+  const ShapeNP<T> A = np[d.shape[i]];
+  const ShapeNP<T> B = np[d.shape[j]];
+  const SimGrid& g = d.grids[d.sim[i]];
+  const V3<double> xi = load3(d.pos, n, i);
+  Tr<T> a2w, b2w;
+  pair_frames<T>(g, xi, load_quat(d.quat, n, i), load3(d.pos, n, j), load_quat(d.quat, n, j), a2w, b2w);
+  const GjkArgs args = {d.flags, d.gjk_rel};
+  CInfo<T> ci = closest_points<T>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
+  uint32_t flag = (st.gjk ? kFlagGjk : 0u) | (st.deep ? kFlagDeep : 0u);
+  if (ci.overlap < T(0)) {
+    flag |= kFlagContact;
+    const HookeP<T> h = hk_table<T>(d)[g.table + material_hash((uint32_t)A.material, (uint32_t)B.material)];
+    const V3<T> vi = cvt<T>(load3(d.vel, n, i)), wi = cvt<T>(load3(d.omega, n, i));
+    const V3<T> vj = cvt<T>(load3(d.vel, n, j)), wj = cvt<T>(load3(d.omega, n, j));
+    V3<T> li, lj;
+    if (d.flags & GRAINS_REF_QUIRKS) {
+      // reference lever: the world contact point (Kinematics.cpp:86-91)
+      V3<T> cpw;
+      if constexpr (sizeof(T) == 8)
+        cpw = ci.cp;
+      else
+        cpw = cvt<float>(xi + cvt<double>(ci.cp));
+      li = cpw;
+      lj = cpw;
+    } else {
+      li = ci.cp - a2w.o;
+      lj = ci.cp - b2w.o;
+    }
+    const V3<T> relVel = (vi + cross(wi, li)) - (vj + cross(wj, lj));
+    const V3<T> relAng = wi - wj;
+    V3<T> F, M;
+    hooke<T>(h, ci, relVel, relAng, A.mass, B.mass, ci.cp - a2w.o, F, M);
+    out.f[0] = R(F.x);
+    out.f[1] = R(F.y);
+    out.f[2] = R(F.z);
+    out.m[0] = R(M.x);
+    out.m[1] = R(M.y);
+    out.m[2] = R(M.z);
+  }
+  return flag;
+}
+
+// One lane per candidate entry, grid-stride over the E entries of this step.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
+  const uint32_t E = d.entry_off[d.n];
+  PairRes<T>* res = (PairRes<T>*)d.pair_res;
+  uint32_t c_gjk = 0, c_it = 0, c_con = 0, c_deep = 0, c_ref = 0, c_bk = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+    NpStats st = {0, 0, 0, 0};
+    PairRes<T> r;
+    uint32_t flag = pair_force<T, T>(d, d.pair_i[e], d.pair_j[e], r, st);
+    if (flag & kFlagContact) res[e] = r;
+    if constexpr (sizeof(T) == 4) {
+      // FP32-mixed: re-run in FP64 when the sub-algorithm fell back, the
+      // cores overlap, or the FP32 result cannot decide contact
+      bool refine = st.backup || st.deep;
+      if (refine) {
+        flag |= kFlagRefine;
+        uint32_t slot = atomicAdd(d.refine_count, 1u);
+        d.refine_list[slot] = (uint32_t)e;
+        ++c_ref;
+      }
+    }
+    d.pair_flag[e] = flag;
+    c_gjk += st.gjk;
+    c_it += st.iters;
+    c_con += (flag & kFlagContact) ? 1u : 0u;
+    c_deep += st.deep;
+    c_bk += st.backup;
+  }
+  count_add(&d.counters[kCntGjk], c_gjk);
+  count_add(&d.counters[kCntGjkIters], c_it);
+  count_add(&d.counters[kCntContacts], c_con);
+  count_add(&d.counters[kCntDeep], c_deep);
+  count_add(&d.counters[kCntRefine], c_ref);
+  count_add(&d.counters[kCntBackup], c_bk);
+}
+
+// FP64 re-run of the entries queued by k_narrow<float>; results stored in FP32.
+__global__ void __launch_bounds__(kBlock) k_refine64(DevScene d);
+
+// Per particle: sum its entries in visit order, walls, gravity, then
+// moveParticles (ComponentManagerCPU.cpp:428-500) on the quaternion state.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
+  const uint64_t n = d.n;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t walls = 0;
+  if (i < n) {
+    const PairRes<T>* res = (const PairRes<T>*)d.pair_res;
+    const uint32_t s = d.rank[i];
+    const uint32_t e0 = d.entry_off[s], e1 = d.entry_off[s + 1];
+    V3<double> F = {0.0, 0.0, 0.0}, M = {0.0, 0.0, 0.0};
+    int32_t ncon = 0;
+    for (uint32_t e = e0; e < e1; ++e) {
+      if (d.pair_flag[e] & kFlagContact) {
+        const PairRes<T> r = res[e];
+        F = F + V3<double>{(double)r.f[0], (double)r.f[1], (double)r.f[2]};
+        M = M + V3<double>{(double)r.m[0], (double)r.m[1], (double)r.m[2]};
+        ++ncon;
+      }
+    }
+    const uint32_t si = d.shape[i];
+    const SimGrid& g = d.grids[d.sim[i]];
+    const V3<double> xi = load3(d.pos, n, i);
+    const Qt<double> qi = load_quat(d.quat, n, i);
+    const V3<double> vel = load3(d.vel, n, i);
+    const V3<double> om = load3(d.omega, n, i);
+    // particle x obstacle (ComponentManagerCPU.cpp:301-351)
+    if (g.obs_end > g.obs_begin) {
+      const ShapeNP<T> A = np_table<T>(d)[si];
+      for (uint32_t o = g.obs_begin; o < g.obs_end; ++o) {
+        const ShapeNP<T> B = np_table<T>(d)[d.obs_shape[o]];
+        const double* t = d.obs_tr + 12 * (size_t)o;
+        Tr<T> a2w, b2w;
+        M3<double> Ro;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Ro.m[k] = t[k];
+        const V3<double> xo = {t[9], t[10], t[11]};
+        if constexpr (sizeof(T) == 8) {
+          a2w.R = rot(qi);
+          a2w.o = xi;
+          b2w.R = Ro;
+          b2w.o = xo;
+        } else {
+          a2w.R = cvt<float>(rot(qi));
+          a2w.o = {0.0f, 0.0f, 0.0f};
+          b2w.R = cvt<float>(Ro);
+          b2w.o = cvt<float>(xo - xi);
+        }
+        // obstacles are Rectangle walls (checked on the host): the plane branch
+        // of closestPointsRigidBodies (CollisionDetection.cpp:203-206)
+        CInfo<T> ci = cp_rectangle(sup_ctx(A, false, vt_full<T>(d), vt_core<T>(d)), a2w, b2w);
+        if (ci.overlap < T(0)) {
+          const HookeP<T> h = hk_table<T>(d)[g.table + material_hash((uint32_t)A.material, (uint32_t)B.material)];
+          V3<T> lever;
+          if (d.flags & GRAINS_REF_QUIRKS) {
+            if constexpr (sizeof(T) == 8)
+              lever = ci.cp;
+            else
+              lever = cvt<float>(xi + cvt<double>(ci.cp));
+          } else {
+            lever = ci.cp - a2w.o;
+          }
+          const V3<T> wi = cvt<T>(om);
+          const V3<T> relVel = cvt<T>(vel) + cross(wi, lever);
+          V3<T> f, m;
+          hooke<T>(h, ci, relVel, wi, A.mass, B.mass, ci.cp - a2w.o, f, m);
+          F = F + cvt<double>(f);
+          M = M + cvt<double>(m);
+          ++walls;
+        }
+      }
+    }
+    // addExternalForces (ComponentManagerCPU.cpp:448-460)
+    const ShapeDyn dy = d.dyn[si];
+    F = F + dy.mass * V3<double>{g.g[0], g.g[1], g.g[2]};
+    store3(d.last_force, n, i, F);
+    store3(d.last_torque, n, i, M);
+    d.contact_count[i] += ncon;
+    // RigidBody::computeMomentum (RigidBody.cpp:267-307) with the quaternion state
+    const Qt<double> qc = conj(qi);
+    const V3<double> av = mult_to_v3(qc, vq(om, qi));
+    V3<double> am = mult_to_v3(qc, vq(M, qi));
+    const double* I = dy.inertia;
+    const double* Ii = dy.inv;
+    V3<double> tmp = {I[0] * av.x + I[1] * av.y + I[2] * av.z, I[1] * av.x + I[3] * av.y + I[4] * av.z,
+                      I[2] * av.x + I[4] * av.y + I[5] * av.z};
+    am = am + cross(tmp, av);
+    tmp = {Ii[0] * am.x + Ii[1] * am.y + Ii[2] * am.z, Ii[1] * am.x + Ii[3] * am.y + Ii[4] * am.z,
+           Ii[2] * am.x + Ii[4] * am.y + Ii[5] * am.z};
+    const V3<double> alpha = mult_to_v3(qi, vq(tmp, qc));
+    const V3<double> accel = F / dy.mass;
+    // FirstOrderExplicit::Move (FirstOrderExplicit.cpp:46-60)
+    const double dt = d.dt;
+    const V3<double> trans = dt * vel;
+    const V3<double> vnew = vel + dt * accel;
+    Qt<double> dq = {0.0, 0.0, 0.0, 1.0};
+    const double nO = norm(om);
+    if (nO > Eps<double>::HIGH) {
+      // TimeIntegrator::computeQuaternionChange (TimeIntegrator.cpp:28-44)
+      const double c = cos(nO * dt / 2.0);
+      const double sn = sin(nO * dt / 2.0);
+      dq = qmake((sn / nO) * om, c);
+    }
+    const V3<double> onew = om + dt * alpha;
+    // Transform3::updateTransform (Transform3.cpp:323-330): translate, then
+    // rotate on the left by dq -- on the quaternion: q <- dq x q
+    V3<double> x = xi + trans;
+    const Qt<double> qn = qmul(dq, qi);
+    if (g.periodic) {
+      // periodic wrap of the centre (not in the reference)
+      auto wrap = [&](double c, int k) {
+        if (!((g.periodic >> k) & 1)) return c;
+        if (c < g.lo[k]) return c + g.L[k];
+        if (c >= g.hi[k]) return c - g.L[k];
+        return c;
+      };
+      x = {wrap(x.x, 0), wrap(x.y, 1), wrap(x.z, 2)};
+    }
+    store3(d.pos, n, i, x);
+    store3(d.vel, n, i, vnew);
+    store3(d.omega, n, i, onew);
+    d.quat[i] = qn.x;
+    d.quat[n + i] = qn.y;
+    d.quat[2 * n + i] = qn.z;
+    d.quat[3 * n + i] = qn.w;
+  }
+  count_add(&d.counters[kCntWall], walls);
+}
+
+// GrainsTestDev workload: GJK distance of independent pairs of full shapes.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_gjk_batch(const ShapeNP<T>* __restrict__ shapes,
+                                                     const T* __restrict__ verts, const uint32_t* __restrict__ sa,
+                                                     const uint32_t* __restrict__ sb, const double* __restrict__ ta,
+                                                     const double* __restrict__ tb, uint64_t n, GjkArgs args,
+                                                     double* __restrict__ dist, double* __restrict__ pa,
+                                                     double* __restrict__ pb, int32_t* __restrict__ iters) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double* A12 = ta + 12 * k;
+  const double* B12 = tb + 12 * k;
+  Tr<T> a2w, b2w;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    a2w.R.m[c] = T(A12[c]);
+    b2w.R.m[c] = T(B12[c]);
+  }
+  const V3<double> xa = {A12[9], A12[10], A12[11]}, xb = {B12[9], B12[10], B12[11]};
+  if constexpr (sizeof(T) == 8) {
+    a2w.o = xa;
+    b2w.o = xb;
+  } else {
+    a2w.o = {0.0f, 0.0f, 0.0f};
+    b2w.o = cvt<float>(xb - xa);
+  }
+  SupCtx<T> ca = sup_ctx(shapes[sa[k]], false, verts, verts);
+  SupCtx<T> cb = sup_ctx(shapes[sb[k]], false, verts, verts);
+  GjkOut<T> g = gjk_jh(ca, cb, a2w, b2w, args);
+  dist[k] = (double)g.dist;
+  if (iters) iters[k] = g.iters;
+  if (pa) {
+    V3<double> p = cvt<double>(apply(a2w, g.pa)), q = cvt<double>(apply(b2w, g.pb));
+    if constexpr (sizeof(T) == 4) {
+      p = p + xa;
+      q = q + xa;
+    }
+    pa[3 * k] = p.x;
+    pa[3 * k + 1] = p.y;
+    pa[3 * k + 2] = p.z;
+    pb[3 * k] = q.x;
+    pb[3 * k + 1] = q.y;
+    pb[3 * k + 2] = q.z;
+  }
+}
+
+}  // namespace gr

@@ -1,0 +1,20 @@
+// gr_narrow32.hip -- FP32-mixed narrow phase (GJK, support maps, Hooke in
+// FP32, relative frames taken in FP64), FP64 accumulation and integration.
+#include "gr_narrow.hpp"
+
+namespace gr {
+
+void launch_narrow32(const DevScene& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_narrow<float>, dim3(persistent_blocks()), dim3(kBlock), 0, s, d);
+}
+void launch_integrate32(const DevScene& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_integrate<float>, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d);
+}
+void launch_gjk_batch32(const ShapeNP<float>* shapes, const float* verts, const uint32_t* sa,
+                        const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,
+                        double* dist, double* pa, double* pb, int32_t* iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_gjk_batch<float>, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                     shapes, verts, sa, sb, ta, tb, n, args, dist, pa, pb, iters);
+}
+
+}  // namespace gr

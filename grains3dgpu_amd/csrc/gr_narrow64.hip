@@ -1,0 +1,39 @@
+// gr_narrow64.hip -- FP64 ("plain") narrow phase, integration, GJK batch and
+// the FP64 re-run used by the FP32-mixed path.  Built with
+// -ffp-contract=off so every product and sum rounds as in the reference CPU
+// code (and the oracle).
+#include "gr_narrow.hpp"
+
+namespace gr {
+
+__global__ void __launch_bounds__(kBlock) k_refine64(DevScene d) {
+  const uint32_t cnt = *d.refine_count;
+  PairRes<float>* res = (PairRes<float>*)d.pair_res;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += stride) {
+    const uint32_t e = d.refine_list[k];
+    NpStats st = {0, 0, 0, 0};
+    PairRes<float> r;
+    uint32_t flag = pair_force<double, float>(d, d.pair_i[e], d.pair_j[e], r, st);
+    if (flag & kFlagContact) res[e] = r;
+    d.pair_flag[e] = flag | kFlagRefine;
+  }
+}
+
+void launch_narrow64(const DevScene& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_narrow<double>, dim3(persistent_blocks()), dim3(kBlock), 0, s, d);
+}
+void launch_refine64(const DevScene& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_refine64, dim3(persistent_blocks() / 4), dim3(kBlock), 0, s, d);
+}
+void launch_integrate64(const DevScene& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_integrate<double>, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d);
+}
+void launch_gjk_batch64(const ShapeNP<double>* shapes, const double* verts, const uint32_t* sa,
+                        const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,
+                        double* dist, double* pa, double* pb, int32_t* iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_gjk_batch<double>, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                     shapes, verts, sa, sb, ta, tb, n, args, dist, pa, pb, iters);
+}
+
+}  // namespace gr

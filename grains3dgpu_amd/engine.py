@@ -1,0 +1,206 @@
+"""Python driver of libgrains_hip.so (ctypes over include/grains_hip.h).
+
+`Engine` mirrors the reference's ComponentManager usage from the driver loop
+(Grains/Base/src/GrainsCPU.cpp:46-70): set the scene, step, read the state.
+There is no CPU fallback: if the HIP library is missing or no device is
+present, construction raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi as A
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgrains_hip.so")
+_lib = None
+
+vp = C.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc for gfx950)")
+        L = C.CDLL(LIB_PATH)
+        L.grains_version.restype = C.c_char_p
+        L.grains_create.argtypes = [C.POINTER(A.Config), C.POINTER(vp)]
+        L.grains_destroy.argtypes = [vp]
+        L.grains_last_error.restype = C.c_char_p
+        L.grains_last_error.argtypes = [vp]
+        L.grains_set_shapes.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32]
+        L.grains_set_contact_tables.argtypes = [vp, vp, C.c_uint32, C.c_uint32]
+        L.grains_set_sims.argtypes = [vp, vp, C.c_uint32]
+        L.grains_set_obstacles.argtypes = [vp, vp, vp, vp, C.c_uint32]
+        L.grains_set_particles.argtypes = [vp, vp, vp, C.c_uint64]
+        L.grains_set_state.argtypes = [vp, vp, vp, vp, vp, C.c_uint64]
+        L.grains_get_state.argtypes = [vp, vp, vp, vp, vp, C.c_uint64]
+        L.grains_set_transforms.argtypes = [vp, vp, C.c_uint64]
+        L.grains_get_transforms.argtypes = [vp, vp, C.c_uint64]
+        L.grains_step.argtypes = [vp, C.c_uint32]
+        L.grains_step_graph.argtypes = [vp, C.c_uint32]
+        L.grains_get_cell_keys.argtypes = [vp, vp, vp, C.c_uint64]
+        L.grains_get_pairs.argtypes = [vp, C.c_int, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.grains_get_contact_count.argtypes = [vp, vp, C.c_uint64]
+        L.grains_reset_contact_count.argtypes = [vp]
+        L.grains_get_torce.argtypes = [vp, vp, vp, C.c_uint64]
+        L.grains_get_stats.argtypes = [vp, C.POINTER(A.Stats)]
+        L.grains_gjk_batch.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp, vp, vp]
+        L.grains_gjk_batch_device.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp,
+                                              C.POINTER(C.c_float)]
+        L.grains_snapshot.argtypes = [vp, vp, C.POINTER(C.c_size_t)]
+        L.grains_restore.argtypes = [vp, vp, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _c(a, dtype=np.float64):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+class GrainsError(RuntimeError):
+    pass
+
+
+class Engine:
+    """One grains_ctx_t.  precision: _abi.FP64 | _abi.FP32_MIXED."""
+
+    def __init__(self, dt=1e-6, precision=A.FP64, flags=0, gjk_rel_tol=0.0, max_neighbors=0,
+                 block_size=0):
+        self.L = lib()
+        cfg = A.Config()
+        cfg.dt, cfg.precision, cfg.flags, cfg.gjk_rel_tol = dt, precision, flags, gjk_rel_tol
+        cfg.max_neighbors, cfg.block_size = max_neighbors, block_size
+        h = vp()
+        rc = self.L.grains_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise GrainsError(f"grains_create failed ({rc}): no HIP device or bad config")
+        self.h = h
+        self.n = 0
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.grains_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise GrainsError(f"grains error {rc}: {self.L.grains_last_error(self.h).decode()}")
+
+    # scene ----------------------------------------------------------------------
+    def set_shapes(self, sc):
+        arr = sc.shape_array()
+        v = _c(sc.vertices.reshape(-1, 3)) if len(sc.vertices) else None
+        self._keep += [arr, v]
+        self._chk(self.L.grains_set_shapes(self.h, C.cast(arr, vp), len(sc.shapes), A.dptr(v),
+                                           0 if v is None else v.shape[0]))
+
+    def set_contact_tables(self, sc):
+        arr = sc.table_array()
+        self._chk(self.L.grains_set_contact_tables(self.h, C.cast(arr, vp), sc.n_materials, sc.n_tables))
+
+    def set_sims(self, sc):
+        arr = sc.sim_array()
+        self._chk(self.L.grains_set_sims(self.h, C.cast(arr, vp), len(sc.sims)))
+
+    def set_obstacles(self, sc):
+        sh = _c(sc.obstacle_shape, np.uint32)
+        sm = _c(sc.obstacle_sim, np.uint32)
+        tr = _c(sc.obstacle_tr)
+        self._chk(self.L.grains_set_obstacles(self.h, A.dptr(sh), A.dptr(sm), A.dptr(tr), len(sh)))
+
+    def set_particles(self, sc):
+        sh = _c(sc.particle_shape, np.uint32)
+        sm = _c(sc.particle_sim, np.uint32)
+        self.n = len(sh)
+        self._chk(self.L.grains_set_particles(self.h, A.dptr(sh), A.dptr(sm), self.n))
+
+    def set_state(self, pos=None, quat=None, vel=None, omega=None):
+        p, q, v, o = _c(pos), _c(quat), _c(vel), _c(omega)
+        self._chk(self.L.grains_set_state(self.h, A.dptr(p), A.dptr(q), A.dptr(v), A.dptr(o), self.n))
+
+    def set_transforms(self, tr):
+        t = _c(tr)
+        self._chk(self.L.grains_set_transforms(self.h, A.dptr(t), self.n))
+
+    # run / results ------------------------------------------------------------------
+    def step(self, n_steps=1, graph=False):
+        f = self.L.grains_step_graph if graph else self.L.grains_step
+        self._chk(f(self.h, n_steps))
+
+    def get_state(self):
+        n = self.n
+        p, q, v, o = np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._chk(self.L.grains_get_state(self.h, A.dptr(p), A.dptr(q), A.dptr(v), A.dptr(o), n))
+        return p, q, v, o
+
+    def get_transforms(self):
+        t = np.zeros((self.n, 12))
+        self._chk(self.L.grains_get_transforms(self.h, A.dptr(t), self.n))
+        return t
+
+    def get_cell_keys(self):
+        k = np.zeros(self.n, np.uint32)
+        s = np.zeros(self.n, np.uint32)
+        self._chk(self.L.grains_get_cell_keys(self.h, A.dptr(k), A.dptr(s), self.n))
+        return k, s
+
+    def get_pairs(self, which=A.PAIRS_CANDIDATE):
+        m = C.c_uint64()
+        self._chk(self.L.grains_get_pairs(self.h, which, None, 0, C.byref(m)))
+        out = np.zeros((m.value, 2), np.uint32)
+        self._chk(self.L.grains_get_pairs(self.h, which, A.dptr(out), m.value, C.byref(m)))
+        return out
+
+    def get_contact_count(self):
+        c = np.zeros(self.n, np.int32)
+        self._chk(self.L.grains_get_contact_count(self.h, A.dptr(c), self.n))
+        return c
+
+    def reset_contact_count(self):
+        self._chk(self.L.grains_reset_contact_count(self.h))
+
+    def get_torce(self):
+        f = np.zeros((self.n, 3))
+        t = np.zeros((self.n, 3))
+        self._chk(self.L.grains_get_torce(self.h, A.dptr(f), A.dptr(t), self.n))
+        return f, t
+
+    def get_stats(self):
+        st = A.Stats()
+        self._chk(self.L.grains_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def snapshot(self):
+        sz = C.c_size_t()
+        self._chk(self.L.grains_snapshot(self.h, None, C.byref(sz)))
+        buf = (C.c_char * sz.value)()
+        self._chk(self.L.grains_snapshot(self.h, buf, C.byref(sz)))
+        return bytes(buf)
+
+    def restore(self, blob):
+        self._chk(self.L.grains_restore(self.h, blob, len(blob)))
+
+    def gjk_batch(self, shape_a, shape_b, ta, tb):
+        sa, sb = _c(shape_a, np.uint32), _c(shape_b, np.uint32)
+        ta, tb = _c(ta), _c(tb)
+        n = len(sa)
+        d = np.zeros(n)
+        pa, pb = np.zeros((n, 3)), np.zeros((n, 3))
+        it = np.zeros(n, np.int32)
+        self._chk(self.L.grains_gjk_batch(self.h, A.dptr(sa), A.dptr(sb), A.dptr(ta), A.dptr(tb), n,
+                                          A.dptr(d), A.dptr(pa), A.dptr(pb), A.dptr(it)))
+        return d, pa, pb, it
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points declared in include/grains_hip.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "grains_hip.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(grains_[a-z_0-9]+)\s*\(", txt)))

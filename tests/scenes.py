@@ -1,0 +1,71 @@
+"""Seeded test scenes shared by the parity tests."""
+import math
+
+import numpy as np
+
+from grains3dgpu_amd import _abi as A
+from grains3dgpu_amd import scene as S
+
+
+def dense_spheres(n=300, seed=5, vel=0.05, omega=10.0, walls=False, periodic=0, etat=2e4):
+    """Random spheres of two sizes at solid fraction 0.45 (many overlaps)."""
+    rng = np.random.default_rng(seed)
+    r = np.where(rng.uniform(size=n) < 0.5, 0.5e-3, 0.35e-3)
+    L = (n * 4 / 3 * np.pi * (r ** 3).mean() / 0.45) ** (1 / 3)
+    pos = rng.uniform(0, L, (n, 3))
+    if walls:
+        pos = 0.6e-3 + rng.uniform(0, L - 1.2e-3, (n, 3))
+    shapes = [dict(kind=A.SPHERE, material=0, dims=(0.5e-3,), density=2500.0, crust=2.5e-5),
+              dict(kind=A.SPHERE, material=0, dims=(0.35e-3,), density=2500.0, crust=2.5e-5)]
+    osh, otr = np.zeros(0, np.uint32), np.zeros((0, 12))
+    if walls:
+        ws, otr = S.box_walls([0, 0, 0], [L, L, L])
+        osh = np.arange(2, 2 + len(ws), dtype=np.uint32)
+        shapes += ws
+    return S.Scene(shapes=shapes, tables=np.array([S.hooke_row(1e5, 0.8, etat, 0.3)]), n_materials=1,
+                   sims=[dict(domain_min=(0, 0, 0), domain_max=(L, L, L), gravity=(0, 0, -9.81),
+                              periodic=periodic)],
+                   particle_shape=(r < 0.4e-3).astype(np.uint32), pos=pos,
+                   quat=S.random_quaternions(rng, n), vel=rng.standard_normal((n, 3)) * vel,
+                   omega=rng.standard_normal((n, 3)) * omega, obstacle_shape=osh, obstacle_tr=otr,
+                   dt=1e-6, name=f"dense_spheres_{n}")
+
+
+def dense_superquadrics(n=300, seed=7, phi=0.25, vel=0.05, omega=10.0, crust_frac=0.08,
+                        periodic=0b011, n_rng=(2.0, 8.0), walls=True, box_frac=0.0):
+    """Random superquadrics (a,b,c in [0.4,1] mm, n1,n2 in n_rng) at random
+    positions: many crust-core overlaps.  Optionally a fraction of boxes."""
+    rng = np.random.default_rng(seed)
+    prm = S.superquadric_params(rng, n, n_rng=n_rng)
+    from grains3dgpu_amd.shapes import superquadric_volume
+    vol = np.array([superquadric_volume(*p) for p in prm])
+    L = (vol.sum() / phi) ** (1 / 3)
+    pos = rng.uniform(0, L, (n, 3))
+    if walls:
+        pos[:, 2] = rng.uniform(1.0e-3, L - 1.0e-3, n)
+    shapes = []
+    for k, p in enumerate(prm):
+        ct = crust_frac * p[:3].min()
+        if rng.uniform() < box_frac:
+            shapes.append(dict(kind=A.BOX, material=0, dims=tuple(p[:3]), crust=ct, density=2500.0))
+        else:
+            shapes.append(dict(kind=A.SUPERQUADRIC, material=0, dims=tuple(p), crust=ct, density=2500.0))
+    osh, otr = np.zeros(0, np.uint32), np.zeros((0, 12))
+    if walls:
+        first = len(shapes)
+        I = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0])
+        shapes += [dict(kind=A.RECTANGLE, dims=(L / 2, L / 2), density=0.0)] * 2
+        osh = np.array([first, first + 1], np.uint32)
+        otr = np.array([np.concatenate([I, [L / 2, L / 2, 0]]), np.concatenate([I, [L / 2, L / 2, L]])])
+    return S.Scene(shapes=shapes, tables=np.array([S.hooke_row(1e5, 0.8, 2e4, 0.3)]), n_materials=1,
+                   sims=[dict(domain_min=(0, 0, 0), domain_max=(L, L, L), gravity=(0, 0, -9.81),
+                              periodic=periodic)],
+                   particle_shape=np.arange(n, dtype=np.uint32), pos=pos,
+                   quat=S.random_quaternions(rng, n), vel=rng.standard_normal((n, 3)) * vel,
+                   omega=rng.standard_normal((n, 3)) * omega, obstacle_shape=osh, obstacle_tr=otr,
+                   dt=1e-6, name=f"dense_superquadrics_{n}")
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))

@@ -1,0 +1,243 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 CPU oracle on the
+same seeded scenes.
+
+Bars: cell keys, sorted order, candidate and contact pair lists and contact
+counts bit-exact; FP64 forces of sphere scenes bit-exact on the first step
+(sqrt and division are correctly rounded on both sides, no FMA contraction);
+FP64 superquadric forces to 1e-9 relative (the pow chains of glibc and the
+device library may differ by an ulp); states after N steps to the stated
+relative tolerances.  FP32-mixed: forces to 2e-3 relative, positions to
+1e-9 relative of the box after N steps.
+"""
+import numpy as np
+import pytest
+
+from grains3dgpu_amd import _abi as A
+from grains3dgpu_amd import scene as S
+from tests.scenes import dense_spheres, dense_superquadrics, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def both(sc, precision=A.FP64, flags=0, **kw):
+    from grains3dgpu_amd import engine
+    from oracle import pyoracle
+    e = S.load(engine.Engine(dt=sc.dt, precision=precision, flags=flags, **kw), sc)
+    o = S.load(pyoracle.Oracle(dt=sc.dt, flags=flags), sc)
+    return e, o
+
+
+def check_broad_phase(e, o):
+    ke, se = e.get_cell_keys()
+    ko, so = o.get_cell_keys()
+    np.testing.assert_array_equal(ke, ko)
+    np.testing.assert_array_equal(se, so)
+    ce, co = e.get_pairs(A.PAIRS_CANDIDATE), o.get_pairs(A.PAIRS_CANDIDATE)
+    np.testing.assert_array_equal(ce, co)
+    return ce
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_spheres_first_step_bit_exact(walls):
+    sc = dense_spheres(n=400, seed=21, walls=walls)
+    e, o = both(sc)
+    e.step(1)
+    o.step(1)
+    cand = check_broad_phase(e, o)
+    assert len(cand) > 500
+    np.testing.assert_array_equal(e.get_pairs(A.PAIRS_CONTACT), o.get_pairs(A.PAIRS_CONTACT))
+    np.testing.assert_array_equal(e.get_contact_count(), o.get_contact_count())
+    fe, te = e.get_torce()
+    fo, to = o.get_torce()
+    np.testing.assert_array_equal(fe, fo)
+    np.testing.assert_array_equal(te, to)
+    pe, _, ve, we = e.get_state()
+    po, _, vo, wo = o.get_state()
+    np.testing.assert_array_equal(pe, po)
+    np.testing.assert_array_equal(ve, vo)
+    assert rel_err(we, wo) < 1e-14
+    np.testing.assert_allclose(e.get_transforms()[:, :9], o.get_transforms()[:, :9], atol=1e-15)
+    st = e.get_stats()
+    assert st["candidate_pairs"] == len(cand) and st["contacts"] == len(e.get_pairs(A.PAIRS_CONTACT))
+
+
+def test_spheres_trajectory_fp64():
+    sc = dense_spheres(n=300, seed=22, walls=True, vel=0.02)
+    e, o = both(sc)
+    for _ in range(4):
+        e.step(50)
+        o.step(50)
+    check_broad_phase(e, o)
+    np.testing.assert_array_equal(e.get_contact_count(), o.get_contact_count())
+    pe, _, ve, we = e.get_state()
+    po, _, vo, wo = o.get_state()
+    assert rel_err(pe, po) < 1e-13
+    assert rel_err(ve, vo) < 1e-9
+    assert rel_err(we, wo) < 1e-9
+
+
+def test_config0_fp64():
+    sc = S.config0()
+    e, o = both(sc)
+    e.step(200)
+    o.step(200)
+    check_broad_phase(e, o)
+    np.testing.assert_array_equal(e.get_contact_count(), o.get_contact_count())
+    pe, _, ve, _ = e.get_state()
+    po, _, vo, _ = o.get_state()
+    assert rel_err(pe, po) < 1e-14
+    assert rel_err(ve, vo) < 1e-12
+
+
+@pytest.mark.parametrize("flags", [0, A.REF_QUIRKS, A.NESTEROV])
+def test_superquadrics_first_step_fp64(flags):
+    sc = dense_superquadrics(n=300, seed=31)
+    e, o = both(sc, flags=flags)
+    e.step(1)
+    o.step(1)
+    check_broad_phase(e, o)
+    ce, co = e.get_pairs(A.PAIRS_CONTACT), o.get_pairs(A.PAIRS_CONTACT)
+    np.testing.assert_array_equal(ce, co)
+    if not flags & A.REF_QUIRKS:
+        assert len(ce) > 50
+    np.testing.assert_array_equal(e.get_contact_count(), o.get_contact_count())
+    fe, te = e.get_torce()
+    fo, to = o.get_torce()
+    assert rel_err(fe, fo) < 1e-9
+    assert rel_err(te, to) < 1e-9
+    se, so = e.get_stats(), o.get_stats()
+    assert se["gjk_calls"] == so["gjk_calls"]
+    assert abs(se["gjk_iterations"] - so["gjk_iterations"]) <= 0.001 * so["gjk_iterations"] + 2
+
+
+def test_superquadrics_trajectory_fp64():
+    sc = dense_superquadrics(n=200, seed=32, vel=0.02)
+    e, o = both(sc)
+    e.step(100)
+    o.step(100)
+    pe, _, ve, _ = e.get_state()
+    po, _, vo, _ = o.get_state()
+    L = sc.sims[0]["domain_max"][0]
+    assert np.abs(pe - po).max() < 1e-12 * L
+    assert rel_err(ve, vo) < 1e-6
+
+
+def test_fp32_mixed_against_fp64_oracle():
+    sc = dense_superquadrics(n=300, seed=33)
+    e, o = both(sc, precision=A.FP32_MIXED)
+    e.step(1)
+    o.step(1)
+    check_broad_phase(e, o)
+    ce, co = e.get_pairs(A.PAIRS_CONTACT), o.get_pairs(A.PAIRS_CONTACT)
+    se, so = set(map(tuple, ce)), set(map(tuple, co))
+    # contact decisions may differ only for pairs at the contact threshold
+    assert len(se ^ so) <= max(2, len(so) // 100)
+    fe, _ = e.get_torce()
+    fo, _ = o.get_torce()
+    ok = np.array([i not in {a for a, _ in se ^ so} for i in range(sc.n)])
+    assert rel_err(fe[ok], fo[ok]) < 2e-3
+    e.step(99)
+    o.step(99)
+    pe, _, _, _ = e.get_state()
+    po, _, _, _ = o.get_state()
+    L = sc.sims[0]["domain_max"][0]
+    assert np.abs(pe - po).max() < 1e-9 * L
+
+
+def test_periodic_spheres():
+    sc = dense_spheres(n=300, seed=41, periodic=0b111)
+    e, o = both(sc)
+    e.step(20)
+    o.step(20)
+    check_broad_phase(e, o)
+    pe, _, _, _ = e.get_state()
+    po, _, _, _ = o.get_state()
+    assert rel_err(pe, po) < 1e-13
+
+
+def test_gjk_batch_matches_oracle():
+    from grains3dgpu_amd import engine
+    from oracle import pyoracle
+    rng = np.random.default_rng(5)
+    n = 256
+    shapes = []
+    for k in range(8):
+        p = S.superquadric_params(rng, 1)[0]
+        shapes.append(dict(kind=A.SUPERQUADRIC, dims=tuple(p), density=1.0))
+    shapes.append(dict(kind=A.BOX, dims=(0.5e-3, 0.3e-3, 0.7e-3), density=1.0))
+    sc = S.Scene(shapes=shapes, tables=np.array([S.hooke_row(1e5, 0.8, 0, 0.3)]), n_materials=1,
+                 sims=[dict(domain_min=(0, 0, 0), domain_max=(1, 1, 1))],
+                 particle_shape=np.zeros(1, np.uint32), pos=np.zeros((1, 3)),
+                 quat=np.array([[0, 0, 0, 1.0]]), vel=np.zeros((1, 3)), omega=np.zeros((1, 3)))
+    sa = rng.integers(0, 9, n).astype(np.uint32)
+    sb = rng.integers(0, 9, n).astype(np.uint32)
+    qa, qb = S.random_quaternions(rng, n), S.random_quaternions(rng, n)
+    off = rng.standard_normal((n, 3))
+    off *= (2.2e-3 + rng.uniform(0.05e-3, 0.5e-3, (n, 1))) / np.linalg.norm(off, axis=1, keepdims=True)
+    ta = S.transform12(rng.uniform(0, 1e-2, (n, 3)), qa)
+    tb = S.transform12(ta[:, 9:] + off, qb)
+    for prec, flags, tol in [(A.FP64, A.REF_QUIRKS, 1e-9), (A.FP64, 0, 1e-9), (A.FP32_MIXED, 0, 2e-4)]:
+        e = engine.Engine(precision=prec, flags=flags)
+        e.set_shapes(sc)
+        d, pa, pb, it = e.gjk_batch(sa, sb, ta, tb)
+        ref = np.array([pyoracle.gjk(shapes[a], shapes[b], x, y, flags=flags)[0]
+                        for a, b, x, y in zip(sa, sb, ta, tb)])
+        assert np.isfinite(d).all()
+        assert rel_err(d, ref) < tol, (prec, flags)
+
+
+def test_multi_sim_batch_matches_separate_oracles():
+    """Two independent simulations in one context (sim id in the cell keys)."""
+    a = dense_spheres(n=150, seed=51)
+    b = dense_spheres(n=220, seed=52, etat=5e3)
+    La, Lb = a.sims[0]["domain_max"][0], b.sims[0]["domain_max"][0]
+    sc = S.Scene(shapes=a.shapes, tables=np.concatenate([a.tables, b.tables]), n_materials=1,
+                 sims=[a.sims[0], dict(b.sims[0], contact_table=1)],
+                 particle_shape=np.concatenate([a.particle_shape, b.particle_shape]),
+                 particle_sim=np.concatenate([np.zeros(a.n, np.uint32), np.ones(b.n, np.uint32)]),
+                 pos=np.concatenate([a.pos, b.pos]), quat=np.concatenate([a.quat, b.quat]),
+                 vel=np.concatenate([a.vel, b.vel]), omega=np.concatenate([a.omega, b.omega]))
+    e, o = both(sc)
+    e.step(10)
+    o.step(10)
+    check_broad_phase(e, o)
+    pe, _, _, _ = e.get_state()
+    po, _, _, _ = o.get_state()
+    assert rel_err(pe, po) < 1e-13
+    # and the batched oracle equals a lone oracle of sim b
+    from oracle import pyoracle
+    ob = S.load(pyoracle.Oracle(dt=b.dt), b)
+    ob.step(10)
+    np.testing.assert_array_equal(po[a.n:], ob.get_state()[0])
+    del La, Lb
+
+
+def test_graph_replay_and_determinism():
+    from grains3dgpu_amd import engine
+    sc = dense_superquadrics(n=200, seed=61)
+    e1 = S.load(engine.Engine(dt=sc.dt), sc)
+    e2 = S.load(engine.Engine(dt=sc.dt), sc)
+    e3 = S.load(engine.Engine(dt=sc.dt), sc)
+    e1.step(30)
+    e2.step(30, graph=True)
+    e3.step(10, graph=True)
+    e3.step(10)
+    e3.step(10, graph=True)
+    s1, s2, s3 = e1.get_state(), e2.get_state(), e3.get_state()
+    for a, b, c in zip(s1, s2, s3):
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(a, c)
+
+
+def test_snapshot_restore_roundtrip():
+    from grains3dgpu_amd import engine
+    sc = dense_spheres(n=200, seed=71)
+    e = S.load(engine.Engine(dt=sc.dt), sc)
+    e.step(5)
+    blob = e.snapshot()
+    e.step(7)
+    after = e.get_state()
+    e.restore(blob)
+    e.step(7)
+    for x, y in zip(after, e.get_state()):
+        np.testing.assert_array_equal(x, y)
