@@ -1,0 +1,276 @@
+#!/usr/bin/env python3
+"""Benchmark of the DEM time step on one MI355X.
+
+Workload: the superquadric bed -- 1,000,000 superquadrics (semi-axes
+U[0.4, 1] mm, blockiness n1, n2 U[2, 8], random orientations, solid fraction
+0.3, periodic in x and y with floor and ceiling walls, Hooke kn = 1e5,
+en = 0.8, mu = 0.3, dt = 1e-6, at rest under gravity).  A "step" is one
+iteration of the reference driver loop (Grains/Base/src/GrainsCPU.cpp:55-66):
+broad phase, narrow phase + contact force, gravity, integration.
+
+Prints ONE JSON line: particle-steps/s of the FP32-mixed path (value), the
+plain FP64 path beside it, the FP32-mixed position error against FP64 after
+the same steps, GJK pair tests/s (in the step and on the reference's GJK
+microbenchmark), the roofline of the dominant kernel measured with HIP events
+over the timed region, and the FP64 CPU oracle timed on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from grains3dgpu_amd import _abi as A  # noqa: E402
+from grains3dgpu_amd import scene as S  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+# Algorithmic operations of the narrow-phase kernel, counted from the source
+# (gr_geom.hpp; transcendental = 1 op, FMA = 2 ops): one GJK iteration = two
+# log-domain superquadric supports (2 x 42), the rotation of the direction
+# and of B's support (33), gap test and bookkeeping (12), Johnson's
+# sub-algorithm (~70 on average over simplex sizes) and |v| (6).
+OPS_PER_GJK_ITER = 205
+# per pair reaching GJK: two rotation matrices (2 x 30), the relative
+# transform (63), OBB test (110), support set-up (2 x 16), witness points and
+# contact vector (45), Hooke force when in contact (counted always, 80)
+OPS_PER_GJK_PAIR = 390
+# per candidate entry: loads, periodic image, bounding-sphere test (20)
+OPS_PER_ENTRY = 20
+# Compulsory HBM bytes per particle-step of the FP32-mixed step at 3 radix
+# passes (state read 84 + write 80, key+index 8, sort 16 x 3, sorted key 4)
+B_PS_FP32 = 224
+B_PS_FP64 = 272
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# GJK relative tolerances of the superquadric bed workload: 1e-10 (FP64),
+# 1e-6 (FP32)
+GJK_TOL = {A.FP64: 1e-10, A.FP32_MIXED: 1e-6}
+
+
+def run_engine(sc, precision, warmup, steps, profile=True, flags=0):
+    from grains3dgpu_amd.engine import Engine
+    e = S.load(Engine(dt=sc.dt, precision=precision, max_neighbors=64, gjk_rel_tol=GJK_TOL[precision],
+                      flags=flags), sc)
+    if warmup:
+        e.step(warmup)
+    e.set_profiling(profile)
+    t0 = time.perf_counter()
+    e.step(steps)
+    t1 = time.perf_counter()
+    st = e.get_stats()
+    ph = e.get_phase_times()
+    pos, quat, vel, om = e.get_state()
+    e.close()
+    return dict(wall_s=t1 - t0, stats=st, phases=ph, pos=pos, vel=vel)
+
+
+def testdev_pairs(n_pairs=614_400, seed=0):
+    """The reference's GJK microbenchmark workload (GrainsTestDev.cpp:75-141):
+    superquadrics r = 0.05, n1 = n2 = 3; centres N(-0.05, 0.005) and
+    N(+0.05, 0.005) per axis; random Euler angles U[0, 2 pi)."""
+    rng = np.random.default_rng(seed)
+
+    def basis(ang):
+        ax, ay, az = ang.T
+        cx, sx, cy, sy, cz, sz = np.cos(ax), np.sin(ax), np.cos(ay), np.sin(ay), np.cos(az), np.sin(az)
+        m = np.stack([cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx,
+                      sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx,
+                      -sy, cy * sx, cy * cx], 1)
+        return m   # Transform3::setBasis(aX, aY, aZ) (Transform3.cpp:151-162)
+
+    ta = np.concatenate([basis(rng.uniform(0, 2 * math.pi, (n_pairs, 3))),
+                         rng.normal(-0.05, 0.005, (n_pairs, 3))], 1)
+    tb = np.concatenate([basis(rng.uniform(0, 2 * math.pi, (n_pairs, 3))),
+                         rng.normal(0.05, 0.005, (n_pairs, 3))], 1)
+    return ta, tb
+
+
+def gjk_microbench(precision):
+    from grains3dgpu_amd.engine import Engine
+    ta, tb = testdev_pairs()
+    n = len(ta)
+    e = Engine(precision=precision, flags=A.REF_QUIRKS)
+    shp = S.Scene(shapes=[dict(kind=A.SUPERQUADRIC, dims=(0.05, 0.05, 0.05, 3.0, 3.0), density=1.0)],
+                  tables=np.array([S.hooke_row(1e5, 0.8, 0, 0.3)]), n_materials=1,
+                  sims=[dict(domain_min=(0, 0, 0), domain_max=(1, 1, 1))],
+                  particle_shape=np.zeros(1, np.uint32), pos=np.zeros((1, 3)),
+                  quat=np.array([[0, 0, 0, 1.0]]), vel=np.zeros((1, 3)), omega=np.zeros((1, 3)))
+    e.set_shapes(shp)
+    z = np.zeros(n, np.uint32)
+    e.gjk_batch(z, z, ta, tb)          # warm-up
+    d, _, _, it = e.gjk_batch(z, z, ta, tb)
+    ms = e.gjk_batch_kernel_ms()
+    e.close()
+    return dict(pairs=n, kernel_ms=ms, pair_tests_per_s=n / (ms * 1e-3), mean_iterations=float(it.mean()),
+                finite=bool(np.isfinite(d).all()))
+
+
+def cpu_baseline(seconds, threads=1):
+    from oracle import pyoracle
+    sc = S.config1(n=8000, seed=7)
+    o = S.load(pyoracle.Oracle(dt=sc.dt, threads=threads), sc)
+    t0 = time.perf_counter()
+    o.step(1)
+    one = time.perf_counter() - t0
+    k = max(1, int(seconds / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    o.step(k)
+    dt = time.perf_counter() - t0
+    st = o.get_stats()
+    o.close()
+    return dict(value=sc.n * k / dt, steps=k, seconds=dt, n=sc.n,
+                gjk_pair_tests_per_s=st["gjk_calls"] / dt)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--fp64-steps", type=int, default=None, help="default: --steps")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the batched multi-simulation form is the way to fill one device", file=sys.stderr)
+        sys.exit(2)
+
+    log(f"generating the superquadric bed, n = {args.n}")
+    t0 = time.perf_counter()
+    sc = S.config1(n=args.n, log=log)
+    log(f"  done in {time.perf_counter() - t0:.1f} s: phi = {sc.notes['phi']:.4f}, "
+        f"rounds = {sc.notes['rounds']}, redraws = {sc.notes['redraws']}")
+
+    K, W = args.steps, args.warmup
+    log(f"FP32-mixed: {W} warm-up + {K} timed steps")
+    r32 = run_engine(sc, A.FP32_MIXED, W, K)
+    ps32 = sc.n * K / r32["wall_s"]
+    log(f"  {r32['wall_s'] * 1e3 / K:.3f} ms/step, {ps32:.3e} particle-steps/s, phases(ms) {r32['phases']}")
+    r64 = None
+    K64 = args.fp64_steps or K
+    if not args.no_fp64:
+        log(f"FP64: {W} warm-up + {K64} timed steps")
+        r64 = run_engine(sc, A.FP64, W, K64)
+        log(f"  {r64['wall_s'] * 1e3 / K64:.3f} ms/step, phases(ms) {r64['phases']}")
+
+    # error of FP32-mixed against FP64 after the same W + K steps
+    err = None
+    if r64 is not None and K64 == K:
+        d = np.abs(r32["pos"] - r64["pos"]).max()
+        disp = np.abs(r64["pos"] - sc.pos).max()
+        err = dict(steps=W + K, pos_rel=float(d / np.abs(r64["pos"]).max()),
+                   pos_rel_displacement=float(d / max(disp, 1e-300)),
+                   vel_rel=float(np.abs(r32["vel"] - r64["vel"]).max() / max(np.abs(r64["vel"]).max(), 1e-300)),
+                   max_displacement_m=float(disp))
+
+    st = r32["stats"]
+    nar_ms = r32["phases"]["narrow"] / K
+    ops = OPS_PER_GJK_ITER * st["gjk_iterations"] / K + OPS_PER_GJK_PAIR * st["gjk_calls"] / K + \
+        OPS_PER_ENTRY * st["candidate_pairs"] / K
+    achieved = ops / (nar_ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "narrow32_c1_1M_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    roofline = dict(bound="valu", kernel="k_narrow<float>", achieved=achieved, peak=PEAK_FP32_TFLOPS,
+                    unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic,
+                    avg_launch_ms=nar_ms, ops_per_launch=ops,
+                    note="narrow phase is VALU/transcendental-bound (no MFMA: no dense contraction); "
+                         "peak = FP32 vector")
+    step_ms = r32["wall_s"] * 1e3 / K
+    roofline_step = dict(bound="hbm", bytes_per_particle_step=B_PS_FP32,
+                         achieved=B_PS_FP32 * ps32 / 1e9, peak=PEAK_HBM_GBS, unit="GB/s",
+                         frac=B_PS_FP32 * ps32 / 1e9 / PEAK_HBM_GBS)
+
+    log("GJK microbenchmark (614,400 superquadric pairs, GrainsTestDev workload)")
+    mb64 = gjk_microbench(A.FP64)
+    mb32 = gjk_microbench(A.FP32_MIXED)
+    log(f"  FP64 {mb64['pair_tests_per_s']:.3e} /s, FP32 {mb32['pair_tests_per_s']:.3e} /s")
+
+    # contact-rich variant for the error of the FP32-mixed path: the same bed
+    # (100k particles) with random velocities (sigma 0.1 m/s) and angular
+    # velocities (sigma 100 rad/s), W + K steps in both precisions
+    agit = None
+    if not args.no_fp64:
+        na = min(args.n, 100_000)
+        sa = S.config1(n=na, seed=11)
+        rng = np.random.default_rng(12)
+        sa.vel = rng.normal(0.0, 0.1, (na, 3))
+        sa.omega = rng.normal(0.0, 100.0, (na, 3))
+        log(f"agitated bed, {na} particles: FP32-mixed and FP64, {W + K} steps each")
+        a32 = run_engine(sa, A.FP32_MIXED, W, K, profile=False)
+        a64 = run_engine(sa, A.FP64, W, K, profile=False)
+        d = np.abs(a32["pos"] - a64["pos"]).max()
+        disp = np.abs(a64["pos"] - sa.pos).max()
+        agit = dict(n=na, steps=W + K, contacts_fp64=a64["stats"]["contacts"],
+                    contacts_fp32=a32["stats"]["contacts"], pos_rel=float(d / np.abs(a64["pos"]).max()),
+                    pos_rel_displacement=float(d / max(disp, 1e-300)), max_displacement_m=float(disp),
+                    vel_rel=float(np.abs(a32["vel"] - a64["vel"]).max() / np.abs(a64["vel"]).max()),
+                    fp32_particle_steps_per_s=na * K / a32["wall_s"],
+                    fp64_particle_steps_per_s=na * K / a64["wall_s"])
+        log(f"  error {agit}")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline: FP64 oracle, 1 thread, ~{args.cpu_seconds:.0f} s")
+        c = cpu_baseline(args.cpu_seconds, threads=1)
+        cpu = dict(value=c["value"], unit="particle-steps/s", cores=1, kind="port",
+                   sample=f"superquadric bed, {c['n']} particles (same distribution and solid fraction), "
+                          f"{c['steps']} steps in {c['seconds']:.1f} s",
+                   gjk_pair_tests_per_s=c["gjk_pair_tests_per_s"])
+        log(f"  {cpu['value']:.3e} particle-steps/s")
+
+    out = {
+        "metric": "particle-steps/s (FP32-mixed) on the superquadric bed",
+        "value": ps32,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "c1 superquadric bed: 1e6 superquadrics a,b,c~U[0.4,1] mm, n1,n2~U[2,8], "
+                               "phi=0.3, periodic x,y + floor/ceiling, dt=1e-6, at rest",
+                   "n_particles": sc.n, "phi": sc.notes["phi"], "precision": "FP32-mixed (FP64 state)"},
+        "fp64": None if r64 is None else {
+            "particle_steps_per_s": sc.n * K64 / r64["wall_s"], "ms_per_step": r64["wall_s"] * 1e3 / K64,
+            "gjk_pair_tests_per_s": r64["stats"]["gjk_calls"] / r64["wall_s"],
+            "speedup_fp32_mixed_over_fp64": ps32 / (sc.n * K64 / r64["wall_s"])},
+        "fp32_mixed_error_vs_fp64": err,
+        "fp32_mixed_error_vs_fp64_agitated": agit,
+        "gjk_pair_tests_per_s": st["gjk_calls"] / r32["wall_s"],
+        "step_stats": {k: st[k] for k in ("candidate_pairs", "gjk_calls", "gjk_iterations", "contacts",
+                                          "wall_contacts", "fp64_refines", "max_candidates", "sort_passes",
+                                          "out_of_domain")},
+        "phase_ms_per_step": {k: v / K for k, v in r32["phases"].items()},
+        "gjk_microbench": {"fp64": mb64, "fp32": mb32},
+        "roofline": roofline,
+        "roofline_step_hbm": roofline_step,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = ps32 / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -62,3 +62,10 @@ def dptr(a):
     if a is None:
         return None
     return a.ctypes.data_as(C.c_void_p)
+
+
+def aptr(a):
+    """Pointer to a ctypes array or a numpy array."""
+    if hasattr(a, "ctypes") and hasattr(a, "dtype"):
+        return a.ctypes.data_as(C.c_void_p)
+    return C.cast(a, C.c_void_p)

@@ -52,6 +52,30 @@ def build(verbose=False):
     return LIB
 
 
+def build_cpp_tests(verbose=False):
+    """C++ programs under tests/cpp that drive the library through
+    grains3dgpu_amd/host/ComponentManagerHIP.hh (built with g++)."""
+    root = os.path.dirname(HERE)
+    src_dir = os.path.join(root, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for f in sorted(os.listdir(src_dir)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, f[:-4])
+        cmd = ["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"), os.path.join(src_dir, f),
+               "-L", HERE, "-lgrains_hip", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../../grains3dgpu_amd",
+               "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed on {f}:\n{r.stderr[-3000:]}")
+        outs.append(exe)
+    if verbose:
+        print("built", outs)
+    return outs
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

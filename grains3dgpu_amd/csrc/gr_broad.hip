@@ -40,15 +40,15 @@ __global__ void k_hash(uint64_t n, const double* __restrict__ pos, const uint32_
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       double f = floor((p[k] - g.lo[k]) / g.h);
-      if (!(f >= 0.0)) {
+      // centres outside [lo, hi] are counted; the partial last cell beyond
+      // n3*h (the domain is rarely a multiple of h) folds into cell n3-1
+      if (!(p[k] >= g.lo[k] && p[k] <= g.hi[k])) ood = 1;
+      if (!(f >= 0.0))
         c[k] = 0;
-        ood = 1;
-      } else if (f >= (double)g.n3[k]) {
+      else if (f >= (double)g.n3[k])
         c[k] = g.n3[k] - 1;
-        ood = 1;
-      } else {
+      else
         c[k] = (uint32_t)f;
-      }
     }
     key[i] = g.base + (c[2] * g.n3[1] + c[1]) * g.n3[0] + c[0];
   }

@@ -58,7 +58,8 @@ double planar_max(double a, double b, double n) {
   if (n > 2.0) {
     double t = std::pow(b / a, n / (n - 2.0));
     double x = a * std::pow(1.0 + std::pow(t * a / b, n), -1.0 / n);
-    r = std::max(r, std::sqrt(x * x + t * x * t * x));
+    double c = std::sqrt(x * x + t * x * t * x);
+    if (std::isfinite(c)) r = std::max(r, c);  // overflow near n = 2: the axis value is the limit
   }
   return r;
 }
@@ -70,7 +71,8 @@ double sq_rc_exact(double a, double b, double c, double n1, double n2) {
     double be = std::pow(c * c / (a * a) * std::pow(T, n1 / n2 - 1.0), 1.0 / (n1 - 2.0));
     double xt = std::pow(std::pow(T, n1 / n2) + std::pow(be, n1), -1.0 / n1);
     double x = a * xt, y = al * b * xt, z = be * c * xt;
-    r = std::max(r, std::sqrt(x * x + y * y + z * z));
+    double q = std::sqrt(x * x + y * y + z * z);
+    if (std::isfinite(q)) r = std::max(r, q);
   }
   return r;
 }
@@ -311,6 +313,12 @@ struct grains_ctx {
   hipGraphExec_t graph_exec = nullptr;
   grains_stats_t stats;
   std::vector<SimGrid> grids;
+  // profiling: events around the phases of each step
+  bool profiling = false;
+  bool capturing = false;
+  std::vector<hipEvent_t> pev;
+  uint32_t pev_used = 0;
+  double phase_ms[GRAINS_NUM_PHASES] = {0};
 };
 
 #define GR_HIP(call)                                                                    \
@@ -344,6 +352,11 @@ static int dalloc(grains_ctx* ctx, P** p, size_t bytes, std::vector<void*>& list
 static void free_list(std::vector<void*>& list) {
   for (void* p : list) (void)hipFree(p);
   list.clear();
+}
+
+static void free_events(grains_ctx* ctx) {
+  for (hipEvent_t e : ctx->pev) (void)hipEventDestroy(e);
+  ctx->pev.clear();
 }
 
 static void drop_graph(grains_ctx* ctx) {
@@ -386,6 +399,7 @@ extern "C" void grains_destroy(grains_ctx_t* ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
+  free_events(ctx);
   free_list(ctx->allocs);
   free_list(ctx->state_allocs);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -773,25 +787,50 @@ extern "C" int grains_get_transforms(grains_ctx_t* ctx, double* t, uint64_t n) {
   return GRAINS_OK;
 }
 
+// phase marker: an event on the stream when profiling (never while capturing)
+static void mark(grains_ctx* ctx) {
+  if (!ctx->profiling || ctx->capturing) return;
+  if (ctx->pev_used == ctx->pev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      ctx->profiling = false;
+      return;
+    }
+    ctx->pev.push_back(e);
+  }
+  (void)hipEventRecord(ctx->pev[ctx->pev_used++], ctx->stream);
+}
+
 // one step, enqueued on ctx->stream (also what a graph captures)
 static void enqueue_step(grains_ctx* ctx) {
   DevScene& d = ctx->d;
   hipStream_t s = ctx->stream;
+  mark(ctx);
   if (d.precision == GRAINS_FP32_MIXED) launch_fill_u32(d.refine_count, 1, 0u, s);
   launch_hash(d, s);
+  mark(ctx);
   launch_sort(d, ctx->passes, &ctx->keys_s, &ctx->idx_s, s);
+  mark(ctx);
   launch_cells(d, ctx->keys_s, ctx->idx_s, s);
+  mark(ctx);
   launch_candidates(d, ctx->keys_s, ctx->idx_s, s);
+  mark(ctx);
   launch_scan(d.cand_count, d.entry_off, d.n, d.scan_blocks, s);
   launch_expand(d, ctx->idx_s, s);
+  mark(ctx);
   if (d.precision == GRAINS_FP64) {
     launch_narrow64(d, s);
+    mark(ctx);
+    mark(ctx);
     launch_integrate64(d, s);
   } else {
     launch_narrow32(d, s);
+    mark(ctx);
     launch_refine64(d, s);
+    mark(ctx);
     launch_integrate32(d, s);
   }
+  mark(ctx);
 }
 
 static int read_stats(grains_ctx* ctx, uint32_t steps) {
@@ -819,11 +858,15 @@ static int run_steps(grains_ctx* ctx, uint32_t n_steps, bool graph) {
   hipStream_t s = ctx->stream;
   launch_fill_u32((uint32_t*)ctx->d.counters, 2 * kNumCounters, 0u, s);
   if (graph && !ctx->graph_exec) {
+    ctx->capturing = true;
     GR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     enqueue_step(ctx);
-    GR_HIP(hipStreamEndCapture(s, &ctx->graph));
+    hipError_t ec = hipStreamEndCapture(s, &ctx->graph);
+    ctx->capturing = false;
+    GR_HIP(ec);
     GR_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
   }
+  ctx->pev_used = 0;
   GR_HIP(hipEventRecord(ctx->ev0, s));
   for (uint32_t k = 0; k < n_steps; ++k) {
     if (graph)
@@ -836,6 +879,16 @@ static int run_steps(grains_ctx* ctx, uint32_t n_steps, bool graph) {
   GR_HIP(hipStreamSynchronize(s));
   float ms = 0;
   GR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  for (int p = 0; p < GRAINS_NUM_PHASES; ++p) ctx->phase_ms[p] = 0;
+  if (ctx->profiling && !graph) {
+    const uint32_t per = GRAINS_NUM_PHASES + 1;
+    for (uint32_t k = 0; k + per <= ctx->pev_used; k += per)
+      for (int p = 0; p < GRAINS_NUM_PHASES; ++p) {
+        float t = 0;
+        GR_HIP(hipEventElapsedTime(&t, ctx->pev[k + p], ctx->pev[k + p + 1]));
+        ctx->phase_ms[p] += t;
+      }
+  }
   rc = read_stats(ctx, n_steps);
   ctx->stats.elapsed_ms = ms;
   if (rc) return rc;
@@ -853,6 +906,18 @@ extern "C" int grains_step(grains_ctx_t* ctx, uint32_t n_steps) {
 extern "C" int grains_step_graph(grains_ctx_t* ctx, uint32_t n_steps) {
   if (!ctx) return GRAINS_EINVAL;
   return run_steps(ctx, n_steps, true);
+}
+
+extern "C" int grains_set_profiling(grains_ctx_t* ctx, int on) {
+  if (!ctx) return GRAINS_EINVAL;
+  ctx->profiling = on != 0;
+  return GRAINS_OK;
+}
+
+extern "C" int grains_get_phase_times(grains_ctx_t* ctx, double* ms, int n) {
+  if (!ctx || !ms || n < 0) return GRAINS_EINVAL;
+  for (int p = 0; p < n && p < GRAINS_NUM_PHASES; ++p) ms[p] = ctx->phase_ms[p];
+  return GRAINS_OK;
 }
 
 extern "C" int grains_get_cell_keys(grains_ctx_t* ctx, uint32_t* keys, uint32_t* sorted, uint64_t n) {
@@ -1030,11 +1095,18 @@ extern "C" int grains_gjk_batch(grains_ctx_t* ctx, const uint32_t* sa, const uin
   (void)hipMemcpy(pv64, v64.data(), v64.size() * 8, hipMemcpyHostToDevice);
   (void)hipMemcpy(pv32, v32.data(), v32.size() * 4, hipMemcpyHostToDevice);
   const GjkArgs args = {ctx->cfg.flags, ctx->cfg.gjk_rel_tol};
+  (void)hipEventRecord(ctx->ev0, ctx->stream);
   if (ctx->cfg.precision == GRAINS_FP64)
     launch_gjk_batch64(p64, pv64, dsa, dsb, dta, dtb, n, args, dd, dpa, dpb, dit, ctx->stream);
   else
     launch_gjk_batch32(p32, pv32, dsa, dsb, dta, dtb, n, args, dd, dpa, dpb, dit, ctx->stream);
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
   hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) {
+    float t = 0;
+    (void)hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+    ctx->stats.elapsed_ms = t;
+  }
   if (e == hipSuccess) {
     (void)hipMemcpy(dist, dd, n * 8, hipMemcpyDeviceToHost);
     if (pa) (void)hipMemcpy(pa, dpa, n * 24, hipMemcpyDeviceToHost);

@@ -404,9 +404,13 @@ struct GjkOut {
 // van den Bergen termination, |v| of the backup simplex, witnesses of the
 // last support pair when no simplex was built; NESTEROV adds momentum on the
 // support direction (Montaut et al. 2022) with the lower bound d.w/|d|.
+// `rsum` = R_A + R_B: with physics semantics every support point and the
+// offset of B are multiplied by the power of two sc, (rsum * sc) in [0.5, 1),
+// so the sub-algorithm's absolute thresholds (det <= EPS) see determinants of
+// order one whatever the unit of length; a power of two scales exactly.
 template <class T>
 GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
-                      const GjkArgs& args) {
+                      const GjkArgs& args, T rsum) {
   const bool quirks = (args.flags & GRAINS_REF_QUIRKS) != 0;
   const bool nesterov = !quirks && (args.flags & GRAINS_NESTEROV);
   const T rel = args.rel > 0 ? T(args.rel) : Eps<T>::LOW;
@@ -426,7 +430,14 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   int last = 0;
   T mu = T(0);
   int num_iter = 0;
-  const Tr<T> b2a = relative(a2w, b2w);
+  Tr<T> b2a = relative(a2w, b2w);
+  T sc = T(1);
+  if (!quirks) {
+    int ex;
+    frexp(rsum, &ex);
+    sc = ldexp(T(1), -ex);
+    b2a.o = sc * b2a.o;
+  }
   const V3<T> zero = {T(0), T(0), T(0)};
   V3<T> v = quirks ? support(A, zero) - apply(b2a, support(B, zero)) : -b2a.o;
   V3<T> w = zero, pl = zero, ql = zero;
@@ -451,8 +462,8 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
         d = (delta / nd) * dir + ((T(1) - delta) / ny) * yk;
       dir = d;
     }
-    pl = support(A, -d);
-    ql = support(B, vm(d, b2a.R));
+    pl = sc * support(A, -d);
+    ql = sc * support(B, vm(d, b2a.R));
     w = pl - apply(b2a, ql);
     ran = true;
     T lb = accel ? dot(d, w) / norm(d) : dot(v, w) / dist;
@@ -491,8 +502,16 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
       default: ok = jh_closest<T, 3>(S, bits, all_bits, v); break;
     }
     if (!ok) {
-      backup = true;
+      // the reference keeps the stale |v| and stops; the physics semantics
+      // use the returned simplex's |v|, and an accelerated search continues
+      // from it as plain GJK
       if (!quirks) dist = norm(v);
+      if (accel) {
+        accel = false;
+        wprev = w;
+        continue;
+      }
+      backup = true;
       break;
     }
     dist = norm(v);
@@ -515,6 +534,12 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   out.iters = num_iter;
   out.deep = !quirks && (bits == 15 || dist <= Eps<T>::HIGH);
   out.backup = backup;
+  if (!quirks) {  // back to the caller's unit of length
+    const T inv = T(1) / sc;
+    out.pa = inv * out.pa;
+    out.pb = inv * out.pb;
+    out.dist = inv * out.dist;
+  }
   return out;
 }
 
@@ -573,7 +598,7 @@ GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<
     if (obb_intersect(A.bbox, B.bbox, a2w, b2w)) {
       SupCtx<T> ca = sup_ctx(A, !quirks, verts_full, verts_core);
       SupCtx<T> cb = sup_ctx(B, !quirks, verts_full, verts_core);
-      GjkOut<T> g = gjk_jh(ca, cb, a2w, b2w, args);
+      GjkOut<T> g = gjk_jh(ca, cb, a2w, b2w, args, A.rc + B.rc);
       st.gjk = 1;
       st.iters = g.iters;
       st.backup = g.backup;

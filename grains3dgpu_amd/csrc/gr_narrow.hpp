@@ -288,9 +288,10 @@ __global__ void __launch_bounds__(kBlock) k_gjk_batch(const ShapeNP<T>* __restri
     a2w.o = {0.0f, 0.0f, 0.0f};
     b2w.o = cvt<float>(xb - xa);
   }
-  SupCtx<T> ca = sup_ctx(shapes[sa[k]], false, verts, verts);
-  SupCtx<T> cb = sup_ctx(shapes[sb[k]], false, verts, verts);
-  GjkOut<T> g = gjk_jh(ca, cb, a2w, b2w, args);
+  const ShapeNP<T> A = shapes[sa[k]], B = shapes[sb[k]];
+  SupCtx<T> ca = sup_ctx(A, false, verts, verts);
+  SupCtx<T> cb = sup_ctx(B, false, verts, verts);
+  GjkOut<T> g = gjk_jh(ca, cb, a2w, b2w, args, A.rc + B.rc);
   dist[k] = (double)g.dist;
   if (iters) iters[k] = g.iters;
   if (pa) {

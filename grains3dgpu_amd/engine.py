@@ -50,6 +50,8 @@ def lib():
         L.grains_gjk_batch.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp, vp, vp]
         L.grains_gjk_batch_device.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp,
                                               C.POINTER(C.c_float)]
+        L.grains_set_profiling.argtypes = [vp, C.c_int]
+        L.grains_get_phase_times.argtypes = [vp, vp, C.c_int]
         L.grains_snapshot.argtypes = [vp, vp, C.POINTER(C.c_size_t)]
         L.grains_restore.argtypes = [vp, vp, C.c_size_t]
         _lib = L
@@ -97,7 +99,7 @@ class Engine:
         arr = sc.shape_array()
         v = _c(sc.vertices.reshape(-1, 3)) if len(sc.vertices) else None
         self._keep += [arr, v]
-        self._chk(self.L.grains_set_shapes(self.h, C.cast(arr, vp), len(sc.shapes), A.dptr(v),
+        self._chk(self.L.grains_set_shapes(self.h, A.aptr(arr), len(sc.shapes), A.dptr(v),
                                            0 if v is None else v.shape[0]))
 
     def set_contact_tables(self, sc):
@@ -176,6 +178,16 @@ class Engine:
         self._chk(self.L.grains_get_stats(self.h, C.byref(st)))
         return st.as_dict()
 
+    PHASES = ("hash", "sort", "cells", "candidates", "scan_expand", "narrow", "refine64", "integrate")
+
+    def set_profiling(self, on=True):
+        self._chk(self.L.grains_set_profiling(self.h, 1 if on else 0))
+
+    def get_phase_times(self):
+        ms = np.zeros(len(self.PHASES))
+        self._chk(self.L.grains_get_phase_times(self.h, A.dptr(ms), len(ms)))
+        return dict(zip(self.PHASES, ms))
+
     def snapshot(self):
         sz = C.c_size_t()
         self._chk(self.L.grains_snapshot(self.h, None, C.byref(sz)))
@@ -196,6 +208,10 @@ class Engine:
         self._chk(self.L.grains_gjk_batch(self.h, A.dptr(sa), A.dptr(sb), A.dptr(ta), A.dptr(tb), n,
                                           A.dptr(d), A.dptr(pa), A.dptr(pb), A.dptr(it)))
         return d, pa, pb, it
+
+    def gjk_batch_kernel_ms(self):
+        """Device time of the last gjk_batch kernel."""
+        return self.get_stats()["elapsed_ms"]
 
 
 def exported_symbols():

@@ -15,6 +15,23 @@ import numpy as np
 from . import _abi as A
 
 
+# numpy view of grains_shape_t (include/grains_hip.h)
+SHAPE_DTYPE = np.dtype([("kind", "<i4"), ("material", "<i4"), ("dims", "<f8", (5,)), ("crust", "<f8"),
+                        ("density", "<f8"), ("n_vertices", "<i4"), ("vertex_offset", "<i4")], align=True)
+
+
+def shape_records(kind, dims, crust, density, material=0):
+    """Structured array of grains_shape_t for many shapes of one kind."""
+    dims = np.atleast_2d(np.asarray(dims, float))
+    r = np.zeros(dims.shape[0], SHAPE_DTYPE)
+    r["kind"] = kind
+    r["material"] = material
+    r["dims"][:, : dims.shape[1]] = dims
+    r["crust"] = crust
+    r["density"] = density
+    return r
+
+
 @dataclass
 class Scene:
     shapes: list                       # list of dict(kind, material, dims, crust, density, n_vertices, vertex_offset)
@@ -41,6 +58,8 @@ class Scene:
 
     # -- ctypes views ------------------------------------------------------
     def shape_array(self):
+        if isinstance(self.shapes, np.ndarray):
+            return np.ascontiguousarray(self.shapes, dtype=SHAPE_DTYPE)
         arr = (A.Shape * max(1, len(self.shapes)))()
         for i, s in enumerate(self.shapes):
             arr[i].kind = s["kind"]
@@ -219,12 +238,12 @@ def superquadric_scene(n, seed=1, spacing_factor=1.0, periodic=0b011, walls_z=Tr
     spacing relative to the largest circumscribed diameter (1.0: bounding
     spheres cannot overlap).  Periodic in x,y (bitmask), floor/ceiling walls
     in z when walls_z."""
-    from .shapes import superquadric_rc
+    from .shapes import superquadric_rc_exact
     rng = np.random.default_rng(seed)
     prm = superquadric_params(rng, n, a_rng=a_rng)
     if n1n2 is not None:
         prm[:, 3:] = n1n2
-    rc = np.array([superquadric_rc(*p) for p in prm]) if n <= 20000 else superquadric_rc_vec(prm)
+    rc = np.array([superquadric_rc_exact(*p) for p in prm])
     sp = spacing_factor * 2.0 * rc.max()
     if lattice is None:
         side = int(math.ceil(n ** (1.0 / 3.0)))
@@ -273,6 +292,93 @@ def superquadric_scene(n, seed=1, spacing_factor=1.0, periodic=0b011, walls_z=Tr
                  name=f"superquadrics_{n}", notes=dict(etat=etat, spacing=sp))
 
 
-def superquadric_rc_vec(prm):
-    from .shapes import superquadric_rc
-    return np.array([superquadric_rc(*p) for p in prm])
+# ---------------------------------------------------------------------------
+def config1(n=1_000_000, phi=0.3, seed=1, jitter=0.05, crust_frac=0.05, resolve=True,
+            max_rounds=40, precision=None, log=None):
+    """Superquadric bed: semi-axes U[0.4, 1] mm, n1, n2 U[2, 8], uniform random
+    orientations, periodic in x and y, floor and ceiling walls, spacing for
+    solid fraction `phi` (by particle volume), kn = 1e5, en = 0.8, mu = 0.3,
+    dt = 1e-6, gravity -9.81 z, at rest.
+
+    Placement: a jittered cubic lattice of spacing (mean volume / phi)^(1/3).
+    Bounding spheres of neighbours overlap at this density, so overlaps of
+    the actual shapes are removed with the library's own narrow phase: each
+    round detects overlapping pairs (crust cores included) and re-draws the
+    orientation -- after 8 rounds the shape too -- of the higher-index
+    particle of every overlapping pair, until no pair overlaps.  The solid
+    fraction actually reached is reported in notes['phi']."""
+    from .shapes import superquadric_volume_vec
+    rng = np.random.default_rng(seed)
+    side = int(round(n ** (1.0 / 3.0)))
+    nx = ny = side
+    nz = int(math.ceil(n / (side * side)))
+    prm = superquadric_params(rng, n)
+    vbar = float(np.mean(superquadric_volume_vec(prm)))
+    sp = (vbar / phi) ** (1.0 / 3.0)
+    idx = np.arange(n)
+    cz, rem = np.divmod(idx, nx * ny)
+    cy, cx = np.divmod(rem, nx)
+    # largest possible circumscribed radius of the distribution (1 mm box corner)
+    margin = 2.0 * math.sqrt(3.0) * 1.0e-3
+    Lx, Ly = nx * sp, ny * sp
+    Lz = nz * sp + 2 * margin
+    pos = np.stack([(cx + 0.5) * sp, (cy + 0.5) * sp, margin + (cz + 0.5) * sp], 1)
+    pos += rng.uniform(-jitter, jitter, pos.shape) * sp
+    pos[:, 0] %= Lx
+    pos[:, 1] %= Ly
+    quat = random_quaternions(rng, n)
+    rho = 2500.0
+    m0 = rho * vbar
+    etat = matched_etat(1e5, 0.8, 0.5 * m0)
+
+    walls = shape_records(A.RECTANGLE, [[0.5 * Lx, 0.5 * Ly]] * 2, 0.0, 0.0)
+    I = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0])
+    obs_tr = np.array([np.concatenate([I, [0.5 * Lx, 0.5 * Ly, 0.0]]),
+                       np.concatenate([I, [0.5 * Lx, 0.5 * Ly, Lz]])])
+
+    def make():
+        sh = np.concatenate([shape_records(A.SUPERQUADRIC, prm, crust_frac * prm[:, :3].min(1), rho), walls])
+        return Scene(shapes=sh, tables=np.array([hooke_row(1e5, 0.8, etat, 0.3, 0.0)]), n_materials=1,
+                     sims=[dict(domain_min=(0.0, 0.0, 0.0), domain_max=(Lx, Ly, Lz), cell_size=0.0,
+                                cell_size_factor=1.0, gravity=(0.0, 0.0, -9.81), periodic=0b011)],
+                     particle_shape=np.arange(n, dtype=np.uint32), pos=pos.copy(), quat=quat.copy(),
+                     vel=np.zeros((n, 3)), omega=np.zeros((n, 3)),
+                     obstacle_shape=np.array([n, n + 1], np.uint32), obstacle_tr=obs_tr, dt=1e-6,
+                     name=f"c1_superquadrics_{n}")
+
+    sc = make()
+    rounds = 0
+    redraws = 0
+    if resolve:
+        from .engine import Engine
+        prec = A.FP32_MIXED if precision is None else precision
+        eng = None
+        shapes_changed = True
+        for rounds in range(1, max_rounds + 1):
+            sc = make()
+            if eng is None:
+                eng = load(Engine(dt=sc.dt, precision=prec, max_neighbors=64), sc)
+            else:
+                if shapes_changed:
+                    eng.set_shapes(sc)
+                eng.set_state(sc.pos, sc.quat, sc.vel, sc.omega)
+            eng.step(1)
+            pairs = eng.get_pairs(A.PAIRS_CONTACT)
+            bad = np.unique(np.maximum(pairs[:, 0], pairs[:, 1])) if len(pairs) else np.zeros(0, int)
+            if log:
+                log(f"config1 round {rounds}: {len(pairs) // 2} overlapping pairs")
+            if len(bad) == 0:
+                break
+            redraws += len(bad)
+            quat[bad] = random_quaternions(rng, len(bad))
+            shapes_changed = rounds >= 8
+            if shapes_changed:
+                prm[bad] = superquadric_params(rng, len(bad))
+        else:
+            raise RuntimeError("config1: overlaps remain after max_rounds")
+        eng.close()
+        sc = make()
+    vol = float(np.sum(superquadric_volume_vec(prm)))
+    sc.notes = dict(etat=etat, spacing=sp, phi=vol / (Lx * Ly * (Lz - 2 * margin)), rounds=rounds,
+                    redraws=redraws, steps=1000)
+    return sc

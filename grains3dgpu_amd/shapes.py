@@ -17,6 +17,15 @@ def superquadric_volume(a, b, c, n1, n2):
     return 2.0 / 3.0 * a * b * c * e1 * e2 * _beta(e1, 0.5 * e1) * _beta(0.5 * e2, 0.5 * e2)
 
 
+def superquadric_volume_vec(prm):
+    """superquadric_volume over rows (a, b, c, n1, n2) of an array."""
+    from scipy.special import betaln
+    prm = np.asarray(prm, float)
+    a, b, c, n1, n2 = prm.T
+    e1, e2 = 2.0 / n1, 2.0 / n2
+    return 2.0 / 3.0 * a * b * c * e1 * e2 * np.exp(betaln(e1, 0.5 * e1) + betaln(0.5 * e2, 0.5 * e2))
+
+
 def superquadric_rc(a, b, c, n1, n2):
     """Superquadric<T>::computeCircumscribedRadius (Superquadric.cpp:122-155)."""
     if n1 == 2.0 and n2 == 2.0:
@@ -35,6 +44,34 @@ def superquadric_rc(a, b, c, n1, n2):
     beta = (gamma * c * c / (a * a)) ** (1.0 / (n1 - 1.0))
     xt = 1.0 / ((1.0 + alpha ** n2) ** (n1 / n2) + beta ** n1) ** (1.0 / n1)
     return math.sqrt(a * xt * a * xt + alpha * b * xt * alpha * b * xt + beta * c * xt * beta * c * xt)
+
+
+def _planar_max(a, b, n):
+    r = max(a, b)
+    if n > 2.0:
+        with np.errstate(all="ignore"):
+            t = np.float64(b / a) ** (n / (n - 2.0))
+            x = a * (1.0 + (t * a / b) ** n) ** (-1.0 / n)
+            c = np.sqrt(x * x + t * x * t * x)
+        if np.isfinite(c):
+            r = max(r, float(c))
+    return r
+
+
+def superquadric_rc_exact(a, b, c, n1, n2):
+    """Largest |p| on the superquadric: maximum over the critical points of
+    |p|^2 of every coordinate family (axes, coordinate planes, interior)."""
+    r = max(_planar_max(a, b, n2), _planar_max(a, c, n1), _planar_max(b, c, n1))
+    if n1 > 2.0 and n2 > 2.0:
+        with np.errstate(all="ignore"):
+            al = np.float64(b / a) ** (2.0 / (n2 - 2.0))
+            T = 1.0 + al ** n2
+            be = (c * c / (a * a) * T ** (n1 / n2 - 1.0)) ** (1.0 / (n1 - 2.0))
+            xt = (T ** (n1 / n2) + be ** n1) ** (-1.0 / n1)
+            q = np.sqrt((a * xt) ** 2 + (al * b * xt) ** 2 + (be * c * xt) ** 2)
+        if np.isfinite(q):
+            r = max(r, float(q))
+    return r
 
 
 def random_hull_points(rng, n_points=8, radius=1.0):

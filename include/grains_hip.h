@@ -212,6 +212,15 @@ int grains_reset_contact_count(grains_ctx_t* ctx);
 int grains_get_torce(grains_ctx_t* ctx, double* force, double* torque, uint64_t n);
 int grains_get_stats(grains_ctx_t* ctx, grains_stats_t* stats);
 
+/* Per-phase device time of the steps run while profiling is on: HIP events
+ * on the context's stream around each kernel group, summed over the steps
+ * of the last grains_step call.  Phases, in order: cell hash, radix sort,
+ * cell ranges, candidates, scan + expand, narrow phase + force, FP64
+ * re-run, reduction + walls + integration. */
+#define GRAINS_NUM_PHASES 8
+int grains_set_profiling(grains_ctx_t* ctx, int on);
+int grains_get_phase_times(grains_ctx_t* ctx, double* ms, int n_phases);
+
 /* Narrow phase alone ------------------------------------------------------- */
 /* GJK distance between shape_a[k] at transform ta[k] and shape_b[k] at tb[k]
  * (12-double Transform3 buffers), the GrainsTestDev microbenchmark
@@ -219,7 +228,8 @@ int grains_get_stats(grains_ctx_t* ctx, grains_stats_t* stats);
  * computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391).  Returns distance, the
  * witness points in world coordinates (may be NULL) and iteration counts
  * (may be NULL).  Honours precision, REF_QUIRKS and NESTEROV of the context;
- * shapes are the full shapes (no crust core). */
+ * shapes are the full shapes (no crust core).  The kernel's device time is
+ * left in grains_stats_t.elapsed_ms. */
 int grains_gjk_batch(grains_ctx_t* ctx, const uint32_t* shape_a,
                      const uint32_t* shape_b, const double* ta,
                      const double* tb, uint64_t n, double* dist, double* pa,

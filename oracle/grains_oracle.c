@@ -259,7 +259,8 @@ static double superellipse_max(double a, double b, double n) {
   if (n > 2.0) {
     double t = pow(b / a, n / (n - 2.0));
     double x = a * pow(1.0 + pow(t * a / b, n), -1.0 / n);
-    r = fmax(r, sqrt(x * x + t * x * t * x));
+    double c = sqrt(x * x + t * x * t * x);
+    if (isfinite(c)) r = fmax(r, c); /* overflow near n = 2: the axis value is the limit */
   }
   return r;
 }
@@ -271,7 +272,8 @@ static double sq_circumscribed_exact(double a, double b, double c, double n1, do
     double beta = pow(c * c / (a * a) * pow(T, n1 / n2 - 1.0), 1.0 / (n1 - 2.0));
     double xt = pow(pow(T, n1 / n2) + pow(beta, n1), -1.0 / n1);
     double x = a * xt, y = alpha * b * xt, z = beta * c * xt;
-    r = fmax(r, sqrt(x * x + y * y + z * z));
+    double q = sqrt(x * x + y * y + z * z);
+    if (isfinite(q)) r = fmax(r, q);
   }
   return r;
 }
@@ -728,6 +730,18 @@ static double gjk_jh(const oshape* A, const oshape* B, const tr3* a2w, const tr3
   const double rel = rel_tol > 0 ? rel_tol : LOWEPS;
   const double abs_err = 1.e-4 * rel;
   tr3 b2a = trelative(a2w, b2w);
+  /* Physics semantics work in the pair's unit frame: every support point and
+   * b2a's offset is multiplied by the power of two sc with
+   * (R_A + R_B) * sc in [0.5, 1).  The sub-algorithm's absolute thresholds
+   * (det <= EPS) then see determinants of order one whatever the unit of
+   * length; a power of two scales without rounding. */
+  double sc = 1.0;
+  if (!quirks) {
+    int e;
+    frexp(A->rc_exact + B->rc_exact, &e);
+    sc = ldexp(1.0, -e);
+    b2a.o = vscale(sc, b2a.o);
+  }
   v3 v;
   if (quirks)
     v = vsub(support(A, V(0, 0, 0), 0), tapply(&b2a, support(B, V(0, 0, 0), 0)));
@@ -755,8 +769,8 @@ static double gjk_jh(const oshape* A, const oshape* B, const tr3* a2w, const tr3
         d = vadd(vscale(delta / nd, dir), vscale((1.0 - delta) / ny, yk));
       dir = d;
     }
-    S.p[last] = support(A, vneg(d), core);
-    S.q[last] = support(B, vm(d, &b2a.R), core);
+    S.p[last] = vscale(sc, support(A, vneg(d), core));
+    S.q[last] = vscale(sc, support(B, vm(d, &b2a.R), core));
     w = vsub(S.p[last], tapply(&b2a, S.q[last]));
     /* optimality gap: lower bound d.w/|d| (= v.w/dist without momentum) */
     double lb = accel ? vdot(d, w) / vnorm(d) : vdot(v, w) / dist;
@@ -777,9 +791,15 @@ static double gjk_jh(const oshape* A, const oshape* B, const tr3* a2w, const tr3
     ++num_iter;
     double dist_before = dist;
     if (!jh_closest(&S, &bits, last, last_bit, all_bits, &v)) {
-      /* backup sub-simplex chosen: the reference keeps the stale |v|;
-       * the physics semantics report the distance of the returned simplex */
+      /* backup sub-simplex chosen: the reference keeps the stale |v| and
+       * stops; the physics semantics use the returned simplex's |v|, and an
+       * accelerated search continues from it as plain GJK */
       if (!quirks) dist = vnorm(v);
+      if (accel) {
+        accel = 0;
+        wprev = w;
+        continue;
+      }
       break;
     }
     dist = vnorm(v);
@@ -794,6 +814,11 @@ static double gjk_jh(const oshape* A, const oshape* B, const tr3* a2w, const tr3
     jh_points(bits, S.det, S.p, S.q, pa, pb);
   if (deep) *deep = (!quirks && (bits == 15 || dist <= HIGHEPS));
   *nb_iter = num_iter;
+  if (!quirks) { /* back to the caller's unit of length */
+    *pa = vscale(1.0 / sc, *pa);
+    *pb = vscale(1.0 / sc, *pb);
+    dist /= sc;
+  }
   return dist;
 }
 
@@ -1083,13 +1108,17 @@ void go_neighbor_table(const uint32_t* n3, uint32_t* table) { neighbor_table_p(n
  * reference converts floor() to uint directly (undefined below 0 and reads
  * out of bounds above n); here each index is clamped into the grid and the
  * point counted as out of domain. */
-static inline uint32_t cell_hash1(const double* min, double h, const uint32_t* n3, v3 p,
-                                  int* ood) {
+static inline uint32_t cell_hash1(const double* min, const double* max, double h,
+                                  const uint32_t* n3, v3 p, int* ood) {
+  double pp[3] = {p.x, p.y, p.z};
   double f[3] = {floor((p.x - min[0]) / h), floor((p.y - min[1]) / h), floor((p.z - min[2]) / h)};
   uint32_t c[3];
   for (int k = 0; k < 3; ++k) {
-    if (!(f[k] >= 0.0)) { c[k] = 0; *ood = 1; }
-    else if (f[k] >= (double)n3[k]) { c[k] = n3[k] - 1; *ood = 1; }
+    /* centres outside [min, max] are counted; the partial last cell beyond
+     * n*h folds into cell n-1 */
+    if (!(pp[k] >= min[k] && pp[k] <= max[k])) *ood = 1;
+    if (!(f[k] >= 0.0)) c[k] = 0;
+    else if (f[k] >= (double)n3[k]) c[k] = n3[k] - 1;
     else c[k] = (uint32_t)f[k];
   }
   return (c[2] * n3[1] + c[1]) * n3[0] + c[0];
@@ -1098,9 +1127,10 @@ static inline uint32_t cell_hash1(const double* min, double h, const uint32_t* n
 uint64_t go_cell_hash(const double* min, double h, const uint32_t* n3, const double* pos,
                       uint64_t n, uint32_t* keys) {
   uint64_t bad = 0;
+  const double max[3] = {min[0] + n3[0] * h, min[1] + n3[1] * h, min[2] + n3[2] * h};
   for (uint64_t i = 0; i < n; ++i) {
     int ood = 0;
-    keys[i] = cell_hash1(min, h, n3, V(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), &ood);
+    keys[i] = cell_hash1(min, max, h, n3, V(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), &ood);
     bad += (uint64_t)ood;
   }
   return bad;
@@ -1278,7 +1308,7 @@ static void update_links(go_world* w) {
   for (uint64_t i = 0; i < w->n; ++i) {
     const osim* S = &w->sims[w->sim[i]];
     int ood = 0;
-    w->keys[i] = S->base + cell_hash1(S->s.domain_min, S->h, S->n3, w->tr[i].o, &ood);
+    w->keys[i] = S->base + cell_hash1(S->s.domain_min, S->s.domain_max, S->h, S->n3, w->tr[i].o, &ood);
     w->pp_ood[i] = (uint64_t)ood;
     w->cell_count[w->keys[i]]++;
   }

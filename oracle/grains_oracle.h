@@ -68,8 +68,8 @@ int go_support(const grains_shape_t* s, const double* vertices,
 void go_cell_counts(const double* min, const double* max, double h, uint32_t* n3);
 /* 27-neighbour table, UINT32_MAX off grid (LinkedCell.cpp:35-72). */
 void go_neighbor_table(const uint32_t* n3, uint32_t* table);
-/* Cell hash of points (LinkedCell.cpp:117-149); returns the number of points
- * whose cell index fell outside the grid (clamped in `keys`). */
+/* Cell hash of points (LinkedCell.cpp:117-149) on the grid [min, min + n*h];
+ * returns the number of points outside it (their indices are clamped). */
 uint64_t go_cell_hash(const double* min, double h, const uint32_t* n3,
                       const double* pos, uint64_t n, uint32_t* keys);
 /* GJK distance of a pair given two 12-double transforms; flags as in

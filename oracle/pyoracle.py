@@ -104,7 +104,7 @@ class Oracle:
         arr = sc.shape_array()
         v = _c(sc.vertices.reshape(-1, 3)) if len(sc.vertices) else None
         self._keep += [arr, v]
-        self._chk(self.L.go_set_shapes(self.w, C.cast(arr, C.c_void_p), len(sc.shapes), A.dptr(v),
+        self._chk(self.L.go_set_shapes(self.w, A.aptr(arr), len(sc.shapes), A.dptr(v),
                                        0 if v is None else v.shape[0]))
 
     def set_contact_tables(self, sc):

@@ -69,3 +69,59 @@ def dense_superquadrics(n=300, seed=7, phi=0.25, vel=0.05, omega=10.0, crust_fra
 def rel_err(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def lattice_spheres(n_side=7, spacing=0.99e-3, jitter=0.02e-3, seed=3, vel=0.02, omega=20.0,
+                    walls=True, etat=2e4, periodic=0):
+    """Spheres (r = 0.5 mm) on a jittered cubic lattice whose neighbours
+    overlap by about 1 % of a diameter: gentle, contact-rich dynamics."""
+    rng = np.random.default_rng(seed)
+    n = n_side ** 3
+    g = np.arange(n_side)
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    L = n_side * spacing
+    pos = (np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + 0.5) * spacing
+    pos += rng.uniform(-jitter, jitter, pos.shape)
+    shapes = [dict(kind=A.SPHERE, material=0, dims=(0.5e-3,), density=2500.0, crust=2.5e-5)]
+    osh, otr = np.zeros(0, np.uint32), np.zeros((0, 12))
+    if walls:
+        ws, otr = S.box_walls([0, 0, 0], [L, L, L])
+        osh = np.arange(1, 1 + len(ws), dtype=np.uint32)
+        shapes += ws
+    return S.Scene(shapes=shapes, tables=np.array([S.hooke_row(1e5, 0.8, etat, 0.3)]), n_materials=1,
+                   sims=[dict(domain_min=(0, 0, 0), domain_max=(L, L, L), gravity=(0, 0, -9.81),
+                              periodic=periodic)],
+                   particle_shape=np.zeros(n, np.uint32), pos=pos, quat=S.random_quaternions(rng, n),
+                   vel=rng.standard_normal((n, 3)) * vel, omega=rng.standard_normal((n, 3)) * omega,
+                   obstacle_shape=osh, obstacle_tr=otr, dt=1e-6, name=f"lattice_spheres_{n}")
+
+
+def lattice_superquadrics(n_side=6, seed=9, gap=0.0, vel=0.02, omega=20.0, crust_frac=0.1,
+                          n_rng=(2.0, 8.0), tilt=0.05):
+    """Superquadrics of semi-axes 0.5 mm and random blockiness on a cubic
+    lattice, tilted by at most `tilt` rad, spaced so neighbours' crusts
+    overlap: gentle contact-rich dynamics."""
+    rng = np.random.default_rng(seed)
+    n = n_side ** 3
+    a = 0.5e-3
+    ct = crust_frac * a
+    ex = rng.uniform(*n_rng, size=(n, 2))
+    sp = 2 * a - 0.5 * ct + gap
+    g = np.arange(n_side)
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    L = n_side * sp
+    pos = (np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + 0.5) * sp
+    axis = rng.standard_normal((n, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    ang = rng.uniform(0, tilt, n)
+    q = np.concatenate([axis * np.sin(ang / 2)[:, None], np.cos(ang / 2)[:, None]], 1)
+    shapes = [dict(kind=A.SUPERQUADRIC, material=0, dims=(a, a, a, e[0], e[1]), crust=ct, density=2500.0)
+              for e in ex]
+    ws, otr = S.box_walls([0, 0, 0], [L, L, L])
+    shapes += ws
+    return S.Scene(shapes=shapes, tables=np.array([S.hooke_row(1e5, 0.8, 2e4, 0.3)]), n_materials=1,
+                   sims=[dict(domain_min=(0, 0, 0), domain_max=(L, L, L), gravity=(0, 0, -9.81))],
+                   particle_shape=np.arange(n, dtype=np.uint32), pos=pos, quat=q,
+                   vel=rng.standard_normal((n, 3)) * vel, omega=rng.standard_normal((n, 3)) * omega,
+                   obstacle_shape=np.arange(n, n + len(ws), dtype=np.uint32), obstacle_tr=otr,
+                   dt=1e-6, name=f"lattice_superquadrics_{n}")
