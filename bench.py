@@ -66,14 +66,24 @@ def run_engine(sc, precision, warmup, steps, profile=True, flags=0):
     if warmup:
         e.step(warmup)
     e.set_profiling(profile)
-    t0 = time.perf_counter()
-    e.step(steps)
-    t1 = time.perf_counter()
-    st = e.get_stats()
-    ph = e.get_phase_times()
+    # timed in at most 10 chunks so a long run keeps reporting progress
+    chunk = max(1, -(-steps // 10))
+    done, wall, phases, stats = 0, 0.0, None, None
+    while done < steps:
+        k = min(chunk, steps - done)
+        t0 = time.perf_counter()
+        e.step(k)
+        wall += time.perf_counter() - t0
+        ph, st = e.get_phase_times(), e.get_stats()
+        phases = ph if phases is None else {p: phases[p] + ph[p] for p in ph}
+        stats = st if stats is None else {c: (max(stats[c], st[c]) if c == "max_candidates" else stats[c] + st[c])
+                                          for c in st}
+        done += k
+        log(f"    {done}/{steps} steps, {wall:.2f} s")
+    t0, t1 = 0.0, wall
     pos, quat, vel, om = e.get_state()
     e.close()
-    return dict(wall_s=t1 - t0, stats=st, phases=ph, pos=pos, vel=vel)
+    return dict(wall_s=t1 - t0, stats=stats, phases=phases, pos=pos, vel=vel)
 
 
 def testdev_pairs(n_pairs=614_400, seed=0):
@@ -140,7 +150,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--fp64-steps", type=int, default=None, help="default: --steps")
+    ap.add_argument("--fp64-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -168,14 +178,15 @@ def main():
         r64 = run_engine(sc, A.FP64, W, K64)
         log(f"  {r64['wall_s'] * 1e3 / K64:.3f} ms/step, phases(ms) {r64['phases']}")
 
-    # error of FP32-mixed against FP64 after the same W + K steps
+    # error of FP32-mixed against FP64 after the same W + K64 steps
     err = None
-    if r64 is not None and K64 == K:
-        d = np.abs(r32["pos"] - r64["pos"]).max()
+    if r64 is not None:
+        p32 = r32 if K64 == K else run_engine(sc, A.FP32_MIXED, W, K64, profile=False)
+        d = np.abs(p32["pos"] - r64["pos"]).max()
         disp = np.abs(r64["pos"] - sc.pos).max()
-        err = dict(steps=W + K, pos_rel=float(d / np.abs(r64["pos"]).max()),
+        err = dict(steps=W + K64, pos_rel=float(d / np.abs(r64["pos"]).max()),
                    pos_rel_displacement=float(d / max(disp, 1e-300)),
-                   vel_rel=float(np.abs(r32["vel"] - r64["vel"]).max() / max(np.abs(r64["vel"]).max(), 1e-300)),
+                   vel_rel=float(np.abs(p32["vel"] - r64["vel"]).max() / max(np.abs(r64["vel"]).max(), 1e-300)),
                    max_displacement_m=float(disp))
 
     st = r32["stats"]

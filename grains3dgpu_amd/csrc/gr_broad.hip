@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_
 }
 
 void launch_candidates(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
-  hipLaunchKernelGGL(k_candidates, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d,
+  hipLaunchKernelGGL(k_candidates, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d,
                      keys_s, idx_s);
 }
 
@@ -339,19 +339,21 @@ __global__ void k_expand(uint64_t n, uint32_t K, const uint32_t* __restrict__ is
 }
 
 void launch_expand(const DevScene& d, const uint32_t* idx_s, hipStream_t s) {
-  hipLaunchKernelGGL(k_expand, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d.n, d.K,
+  hipLaunchKernelGGL(k_expand, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d.n, d.K,
                      idx_s, d.cand, d.cand_count, d.entry_off, d.pair_i, d.pair_j, d.counters);
 }
 
-int persistent_blocks() {
-  static int nb = 0;
-  if (!nb) {
-    int dev = 0, cus = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    nb = (cus > 0 ? cus : 256) * 8;
+int cu_count() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0, c = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+      cus = c;
+    else
+      cus = 256;
   }
-  return nb;
+  return cus;
 }
 
 }  // namespace gr

@@ -378,6 +378,11 @@ extern "C" int grains_create(const grains_config_t* cfg, grains_ctx_t** out) {
   ctx->cfg = *cfg;
   if (ctx->cfg.max_neighbors <= 0) ctx->cfg.max_neighbors = 48;
   if (ctx->cfg.block_size <= 0) ctx->cfg.block_size = 256;
+  if (ctx->cfg.block_size != 64 && ctx->cfg.block_size != 128 && ctx->cfg.block_size != 256) {
+    delete ctx;
+    return GRAINS_EINVAL;
+  }
+  if (ctx->cfg.narrow_blocks_per_cu <= 0) ctx->cfg.narrow_blocks_per_cu = 8;
   std::memset(&ctx->stats, 0, sizeof(ctx->stats));
   std::memset(&ctx->d, 0, sizeof(ctx->d));
   int ndev = 0;
@@ -683,6 +688,8 @@ static int finalize(grains_ctx* ctx) {
   d.flags = ctx->cfg.flags;
   d.gjk_rel = ctx->cfg.gjk_rel_tol;
   d.precision = ctx->cfg.precision;
+  d.block = (uint32_t)ctx->cfg.block_size;
+  d.narrow_blocks = (uint32_t)(cu_count() * ctx->cfg.narrow_blocks_per_cu);
   launch_fill_u32(d.cand_count, n, 0u, ctx->stream);
   launch_fill_u32(d.entry_off, n + 1, 0u, ctx->stream);
   launch_fill_u32(d.refine_count, 4, 0u, ctx->stream);
@@ -906,6 +913,19 @@ extern "C" int grains_step(grains_ctx_t* ctx, uint32_t n_steps) {
 extern "C" int grains_step_graph(grains_ctx_t* ctx, uint32_t n_steps) {
   if (!ctx) return GRAINS_EINVAL;
   return run_steps(ctx, n_steps, true);
+}
+
+extern "C" int grains_set_launch(grains_ctx_t* ctx, int block_size, int narrow_blocks_per_cu) {
+  if (!ctx) return GRAINS_EINVAL;
+  if ((block_size != 64 && block_size != 128 && block_size != 256) || narrow_blocks_per_cu < 1 ||
+      narrow_blocks_per_cu > 64)
+    return fail(ctx, GRAINS_EINVAL, "block size must be 64, 128 or 256 and blocks per CU in [1, 64]");
+  ctx->cfg.block_size = block_size;
+  ctx->cfg.narrow_blocks_per_cu = narrow_blocks_per_cu;
+  ctx->d.block = (uint32_t)block_size;
+  ctx->d.narrow_blocks = (uint32_t)(cu_count() * narrow_blocks_per_cu);
+  drop_graph(ctx);  // a captured step holds the old launch shape
+  return GRAINS_OK;
 }
 
 extern "C" int grains_set_profiling(grains_ctx_t* ctx, int on) {

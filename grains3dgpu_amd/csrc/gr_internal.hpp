@@ -110,6 +110,9 @@ struct DevScene {
   int32_t* contact_count;
   double *last_force, *last_torque;
   unsigned long long* counters;
+  // launch shape of the per-particle / per-entry kernels
+  uint32_t block;               // threads per block (64, 128, 256)
+  uint32_t narrow_blocks;       // blocks of the persistent narrow-phase launch
   // config
   double dt;
   uint32_t flags;
@@ -127,7 +130,7 @@ void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx
 void launch_candidates(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s);
 void launch_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* block_sums, hipStream_t s);
 void launch_expand(const DevScene& d, const uint32_t* idx_s, hipStream_t s);
-int persistent_blocks();
+int cu_count();
 // gr_narrow64.hip / gr_narrow32.hip
 void launch_narrow64(const DevScene& d, hipStream_t s);
 void launch_narrow32(const DevScene& d, hipStream_t s);

@@ -96,7 +96,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   const uint32_t E = d.entry_off[d.n];
   PairRes<T>* res = (PairRes<T>*)d.pair_res;
-  uint32_t c_gjk = 0, c_it = 0, c_con = 0, c_deep = 0, c_ref = 0, c_bk = 0;
+  uint32_t c_gjk = 0, c_it = 0, c_deep = 0, c_ref = 0, c_bk = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
     NpStats st = {0, 0, 0, 0};
@@ -117,13 +117,11 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
     d.pair_flag[e] = flag;
     c_gjk += st.gjk;
     c_it += st.iters;
-    c_con += (flag & kFlagContact) ? 1u : 0u;
     c_deep += st.deep;
     c_bk += st.backup;
   }
   count_add(&d.counters[kCntGjk], c_gjk);
   count_add(&d.counters[kCntGjkIters], c_it);
-  count_add(&d.counters[kCntContacts], c_con);
   count_add(&d.counters[kCntDeep], c_deep);
   count_add(&d.counters[kCntRefine], c_ref);
   count_add(&d.counters[kCntBackup], c_bk);
@@ -138,7 +136,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
   const uint64_t n = d.n;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t walls = 0;
+  uint32_t walls = 0, contacts = 0;
   if (i < n) {
     const PairRes<T>* res = (const PairRes<T>*)d.pair_res;
     const uint32_t s = d.rank[i];
@@ -211,6 +209,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
     store3(d.last_force, n, i, F);
     store3(d.last_torque, n, i, M);
     d.contact_count[i] += ncon;
+    contacts = (uint32_t)ncon;
     // RigidBody::computeMomentum (RigidBody.cpp:267-307) with the quaternion state
     const Qt<double> qc = conj(qi);
     const V3<double> av = mult_to_v3(qc, vq(om, qi));
@@ -260,6 +259,8 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
     d.quat[3 * n + i] = qn.w;
   }
   count_add(&d.counters[kCntWall], walls);
+  // contacts are counted after any FP64 re-run has settled each entry
+  count_add(&d.counters[kCntContacts], contacts);
 }
 
 // GrainsTestDev workload: GJK distance of independent pairs of full shapes.

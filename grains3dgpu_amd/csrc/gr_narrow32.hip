@@ -5,10 +5,10 @@
 namespace gr {
 
 void launch_narrow32(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_narrow<float>, dim3(persistent_blocks()), dim3(kBlock), 0, s, d);
+  hipLaunchKernelGGL(k_narrow<float>, dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
 }
 void launch_integrate32(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_integrate<float>, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d);
+  hipLaunchKernelGGL(k_integrate<float>, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d);
 }
 void launch_gjk_batch32(const ShapeNP<float>* shapes, const float* verts, const uint32_t* sa,
                         const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,

@@ -21,13 +21,13 @@ __global__ void __launch_bounds__(kBlock) k_refine64(DevScene d) {
 }
 
 void launch_narrow64(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_narrow<double>, dim3(persistent_blocks()), dim3(kBlock), 0, s, d);
+  hipLaunchKernelGGL(k_narrow<double>, dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
 }
 void launch_refine64(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_refine64, dim3(persistent_blocks() / 4), dim3(kBlock), 0, s, d);
+  hipLaunchKernelGGL(k_refine64, dim3(d.narrow_blocks / 4 + 1), dim3(kBlock), 0, s, d);
 }
 void launch_integrate64(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_integrate<double>, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d);
+  hipLaunchKernelGGL(k_integrate<double>, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d);
 }
 void launch_gjk_batch64(const ShapeNP<double>* shapes, const double* verts, const uint32_t* sa,
                         const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,

@@ -51,6 +51,7 @@ def lib():
         L.grains_gjk_batch_device.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp,
                                               C.POINTER(C.c_float)]
         L.grains_set_profiling.argtypes = [vp, C.c_int]
+        L.grains_set_launch.argtypes = [vp, C.c_int, C.c_int]
         L.grains_get_phase_times.argtypes = [vp, vp, C.c_int]
         L.grains_snapshot.argtypes = [vp, vp, C.POINTER(C.c_size_t)]
         L.grains_restore.argtypes = [vp, vp, C.c_size_t]
@@ -70,11 +71,12 @@ class Engine:
     """One grains_ctx_t.  precision: _abi.FP64 | _abi.FP32_MIXED."""
 
     def __init__(self, dt=1e-6, precision=A.FP64, flags=0, gjk_rel_tol=0.0, max_neighbors=0,
-                 block_size=0):
+                 block_size=0, narrow_blocks_per_cu=0):
         self.L = lib()
         cfg = A.Config()
         cfg.dt, cfg.precision, cfg.flags, cfg.gjk_rel_tol = dt, precision, flags, gjk_rel_tol
         cfg.max_neighbors, cfg.block_size = max_neighbors, block_size
+        cfg.narrow_blocks_per_cu = narrow_blocks_per_cu
         h = vp()
         rc = self.L.grains_create(C.byref(cfg), C.byref(h))
         if rc != 0:
@@ -179,6 +181,9 @@ class Engine:
         return st.as_dict()
 
     PHASES = ("hash", "sort", "cells", "candidates", "scan_expand", "narrow", "refine64", "integrate")
+
+    def set_launch(self, block_size, narrow_blocks_per_cu):
+        self._chk(self.L.grains_set_launch(self.h, block_size, narrow_blocks_per_cu))
 
     def set_profiling(self, on=True):
         self._chk(self.L.grains_set_profiling(self.h, 1 if on else 0))

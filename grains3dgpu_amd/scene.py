@@ -293,23 +293,15 @@ def superquadric_scene(n, seed=1, spacing_factor=1.0, periodic=0b011, walls_z=Tr
 
 
 # ---------------------------------------------------------------------------
-def config1(n=1_000_000, phi=0.3, seed=1, jitter=0.05, crust_frac=0.05, resolve=True,
-            max_rounds=40, precision=None, log=None):
-    """Superquadric bed: semi-axes U[0.4, 1] mm, n1, n2 U[2, 8], uniform random
-    orientations, periodic in x and y, floor and ceiling walls, spacing for
-    solid fraction `phi` (by particle volume), kn = 1e5, en = 0.8, mu = 0.3,
-    dt = 1e-6, gravity -9.81 z, at rest.
+# superquadric beds (config 1) and batched ensembles of them (config 3)
+_I9 = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0])
 
-    Placement: a jittered cubic lattice of spacing (mean volume / phi)^(1/3).
-    Bounding spheres of neighbours overlap at this density, so overlaps of
-    the actual shapes are removed with the library's own narrow phase: each
-    round detects overlapping pairs (crust cores included) and re-draws the
-    orientation -- after 8 rounds the shape too -- of the higher-index
-    particle of every overlapping pair, until no pair overlaps.  The solid
-    fraction actually reached is reported in notes['phi']."""
+
+def _bed_layout(n, rng, phi, jitter):
+    """Jittered cubic lattice of n superquadrics at solid fraction phi:
+    periodic in x and y, floor and ceiling walls one margin away."""
     from .shapes import superquadric_volume_vec
-    rng = np.random.default_rng(seed)
-    side = int(round(n ** (1.0 / 3.0)))
+    side = max(1, int(round(n ** (1.0 / 3.0))))
     nx = ny = side
     nz = int(math.ceil(n / (side * side)))
     prm = superquadric_params(rng, n)
@@ -327,58 +319,137 @@ def config1(n=1_000_000, phi=0.3, seed=1, jitter=0.05, crust_frac=0.05, resolve=
     pos[:, 0] %= Lx
     pos[:, 1] %= Ly
     quat = random_quaternions(rng, n)
-    rho = 2500.0
-    m0 = rho * vbar
-    etat = matched_etat(1e5, 0.8, 0.5 * m0)
+    return dict(prm=prm, pos=pos, quat=quat, L=(Lx, Ly, Lz), margin=margin, spacing=sp)
 
-    walls = shape_records(A.RECTANGLE, [[0.5 * Lx, 0.5 * Ly]] * 2, 0.0, 0.0)
-    I = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0])
-    obs_tr = np.array([np.concatenate([I, [0.5 * Lx, 0.5 * Ly, 0.0]]),
-                       np.concatenate([I, [0.5 * Lx, 0.5 * Ly, Lz]])])
 
-    def make():
-        sh = np.concatenate([shape_records(A.SUPERQUADRIC, prm, crust_frac * prm[:, :3].min(1), rho), walls])
-        return Scene(shapes=sh, tables=np.array([hooke_row(1e5, 0.8, etat, 0.3, 0.0)]), n_materials=1,
-                     sims=[dict(domain_min=(0.0, 0.0, 0.0), domain_max=(Lx, Ly, Lz), cell_size=0.0,
-                                cell_size_factor=1.0, gravity=(0.0, 0.0, -9.81), periodic=0b011)],
-                     particle_shape=np.arange(n, dtype=np.uint32), pos=pos.copy(), quat=quat.copy(),
-                     vel=np.zeros((n, 3)), omega=np.zeros((n, 3)),
-                     obstacle_shape=np.array([n, n + 1], np.uint32), obstacle_tr=obs_tr, dt=1e-6,
-                     name=f"c1_superquadrics_{n}")
-
-    sc = make()
-    rounds = 0
+def _resolve_overlaps(make, beds, rngs, max_rounds, precision, log, label):
+    """Re-draw, round after round, the orientation -- from round 4 the shape
+    too, from round 15 with semi-axes shrunk by 0.9 per round (not below
+    0.4 mm) -- of the higher-index particle of every overlapping pair, until
+    no pair overlaps.  Overlaps are found with the library's own narrow
+    phase (crust cores included)."""
+    from .engine import Engine
+    prec = A.FP32_MIXED if precision is None else precision
+    offs = np.cumsum([0] + [len(b["prm"]) for b in beds])
+    eng = None
+    shapes_changed = True
     redraws = 0
-    if resolve:
-        from .engine import Engine
-        prec = A.FP32_MIXED if precision is None else precision
-        eng = None
-        shapes_changed = True
-        for rounds in range(1, max_rounds + 1):
-            sc = make()
-            if eng is None:
-                eng = load(Engine(dt=sc.dt, precision=prec, max_neighbors=64), sc)
-            else:
-                if shapes_changed:
-                    eng.set_shapes(sc)
-                eng.set_state(sc.pos, sc.quat, sc.vel, sc.omega)
-            eng.step(1)
-            pairs = eng.get_pairs(A.PAIRS_CONTACT)
-            bad = np.unique(np.maximum(pairs[:, 0], pairs[:, 1])) if len(pairs) else np.zeros(0, int)
-            if log:
-                log(f"config1 round {rounds}: {len(pairs) // 2} overlapping pairs")
-            if len(bad) == 0:
-                break
-            redraws += len(bad)
-            quat[bad] = random_quaternions(rng, len(bad))
-            shapes_changed = rounds >= 8
-            if shapes_changed:
-                prm[bad] = superquadric_params(rng, len(bad))
-        else:
-            raise RuntimeError("config1: overlaps remain after max_rounds")
-        eng.close()
+    for rounds in range(1, max_rounds + 1):
         sc = make()
-    vol = float(np.sum(superquadric_volume_vec(prm)))
-    sc.notes = dict(etat=etat, spacing=sp, phi=vol / (Lx * Ly * (Lz - 2 * margin)), rounds=rounds,
-                    redraws=redraws, steps=1000)
+        if eng is None:
+            eng = load(Engine(dt=sc.dt, precision=prec, max_neighbors=64), sc)
+        else:
+            if shapes_changed:
+                eng.set_shapes(sc)
+            eng.set_state(sc.pos, sc.quat, sc.vel, sc.omega)
+        eng.step(1)
+        pairs = eng.get_pairs(A.PAIRS_CONTACT)
+        bad = np.unique(np.maximum(pairs[:, 0], pairs[:, 1])) if len(pairs) else np.zeros(0, int)
+        if log:
+            log(f"{label} round {rounds}: {len(pairs) // 2} overlapping pairs")
+        if len(bad) == 0:
+            eng.close()
+            return rounds, redraws
+        redraws += len(bad)
+        shapes_changed = rounds >= 4
+        which = np.searchsorted(offs, bad, side="right") - 1
+        for b in np.unique(which):
+            loc = bad[which == b] - offs[b]
+            bed, rng = beds[b], rngs[b]
+            bed["quat"][loc] = random_quaternions(rng, len(loc))
+            if shapes_changed:
+                new = superquadric_params(rng, len(loc))
+                if rounds >= 15:
+                    new[:, :3] = np.maximum(new[:, :3] * 0.9 ** (rounds - 14), 0.4e-3)
+                bed["prm"][loc] = new
+    eng.close()
+    raise RuntimeError(f"{label}: overlaps remain after {max_rounds} rounds")
+
+
+def _bed_scene(beds, tables, crust_frac, rho, name, vel=None, omega=None):
+    """One Scene holding the beds as independent simulations."""
+    shapes, pos, quat, psim, osh, osim, otr, sims = [], [], [], [], [], [], [], []
+    n_tot = sum(len(b["prm"]) for b in beds)
+    for k, b in enumerate(beds):
+        prm = b["prm"]
+        shapes.append(shape_records(A.SUPERQUADRIC, prm, crust_frac * prm[:, :3].min(1), rho))
+        pos.append(b["pos"])
+        quat.append(b["quat"])
+        psim.append(np.full(len(prm), k, np.uint32))
+    first = n_tot
+    walls = []
+    for k, b in enumerate(beds):
+        Lx, Ly, Lz = b["L"]
+        walls.append(shape_records(A.RECTANGLE, [[0.5 * Lx, 0.5 * Ly]] * 2, 0.0, 0.0))
+        osh += [first + 2 * k, first + 2 * k + 1]
+        osim += [k, k]
+        otr += [np.concatenate([_I9, [0.5 * Lx, 0.5 * Ly, 0.0]]), np.concatenate([_I9, [0.5 * Lx, 0.5 * Ly, Lz]])]
+        sims.append(dict(domain_min=(0.0, 0.0, 0.0), domain_max=(Lx, Ly, Lz), cell_size=0.0,
+                         cell_size_factor=b.get("cell_size_factor", 1.0), gravity=(0.0, 0.0, -9.81),
+                         periodic=0b011, contact_table=k if len(tables) > 1 else 0))
+    n = n_tot
+    return Scene(shapes=np.concatenate(shapes + walls), tables=np.array(tables), n_materials=1, sims=sims,
+                 particle_shape=np.arange(n, dtype=np.uint32), particle_sim=np.concatenate(psim),
+                 pos=np.concatenate(pos), quat=np.concatenate(quat),
+                 vel=np.zeros((n, 3)) if vel is None else vel, omega=np.zeros((n, 3)) if omega is None else omega,
+                 obstacle_shape=np.array(osh, np.uint32), obstacle_sim=np.array(osim, np.uint32),
+                 obstacle_tr=np.array(otr), dt=1e-6, name=name)
+
+
+def config1(n=1_000_000, phi=0.3, seed=1, jitter=0.05, crust_frac=0.05, resolve=True,
+            max_rounds=80, precision=None, log=None):
+    """Superquadric bed: semi-axes U[0.4, 1] mm, n1, n2 U[2, 8], uniform random
+    orientations, periodic in x and y, floor and ceiling walls, spacing for
+    solid fraction `phi` (by particle volume), kn = 1e5, en = 0.8, mu = 0.3,
+    dt = 1e-6, gravity -9.81 z, at rest.
+
+    Placement: a jittered cubic lattice of spacing (mean volume / phi)^(1/3).
+    Bounding spheres of neighbours overlap at this density, so overlaps of
+    the actual shapes are removed with the library's own narrow phase
+    (_resolve_overlaps).  The solid fraction reached is notes['phi']."""
+    from .shapes import superquadric_volume_vec
+    rng = np.random.default_rng(seed)
+    bed = _bed_layout(n, rng, phi, jitter)
+    rho = 2500.0
+    m0 = rho * float(np.mean(superquadric_volume_vec(bed["prm"])))
+    etat = matched_etat(1e5, 0.8, 0.5 * m0)
+    tables = [hooke_row(1e5, 0.8, etat, 0.3, 0.0)]
+    make = lambda: _bed_scene([bed], tables, crust_frac, rho, f"c1_superquadrics_{n}")
+    rounds = redraws = 0
+    if resolve:
+        rounds, redraws = _resolve_overlaps(make, [bed], [rng], max_rounds, precision, log, "config1")
+    sc = make()
+    Lx, Ly, Lz = bed["L"]
+    vol = float(np.sum(superquadric_volume_vec(bed["prm"])))
+    sc.notes = dict(etat=etat, spacing=bed["spacing"], phi=vol / (Lx * Ly * (Lz - 2 * bed["margin"])),
+                    rounds=rounds, redraws=redraws, steps=1000)
+    return sc
+
+
+def config3(n_sims=512, n_range=(500, 20_000), seed=3, phi=0.3, jitter=0.05, crust_frac=0.05,
+            resolve=True, max_rounds=80, precision=None, log=None, cell_size_factor=1.0):
+    """Batched ensemble: n_sims independent superquadric beds (as config1,
+    each with its own seed) of log-uniform sizes in n_range, kn ~ U[5e4, 5e5],
+    mu ~ U[0.1, 0.5], one simulation each in one context."""
+    from .shapes import superquadric_volume_vec
+    master = np.random.default_rng(seed)
+    sizes = np.exp(master.uniform(np.log(n_range[0]), np.log(n_range[1]), n_sims)).astype(int)
+    kn = master.uniform(5e4, 5e5, n_sims)
+    mu = master.uniform(0.1, 0.5, n_sims)
+    rngs = [np.random.default_rng(seed * 100003 + k) for k in range(n_sims)]
+    beds = [_bed_layout(int(sz), r, phi, jitter) for sz, r in zip(sizes, rngs)]
+    for b in beds:
+        b["cell_size_factor"] = cell_size_factor
+    rho = 2500.0
+    tables = []
+    for k, b in enumerate(beds):
+        m0 = rho * float(np.mean(superquadric_volume_vec(b["prm"])))
+        tables.append(hooke_row(kn[k], 0.8, matched_etat(kn[k], 0.8, 0.5 * m0), mu[k], 0.0))
+    make = lambda: _bed_scene(beds, tables, crust_frac, rho, f"c3_ensemble_{n_sims}")
+    rounds = redraws = 0
+    if resolve:
+        rounds, redraws = _resolve_overlaps(make, beds, rngs, max_rounds, precision, log, "config3")
+    sc = make()
+    sc.notes = dict(sizes=sizes.tolist(), kn=kn.tolist(), mu=mu.tolist(), rounds=rounds, redraws=redraws,
+                    steps=1000)
     return sc

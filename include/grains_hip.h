@@ -119,9 +119,11 @@ typedef struct grains_config_t {
   uint32_t flags;           /* GRAINS_REF_QUIRKS | GRAINS_NESTEROV           */
   double gjk_rel_tol;       /* 0: LOWEPS of the precision (Basic.hh:126-128) */
   int32_t max_neighbors;    /* candidate capacity per particle, 0: 48        */
-  int32_t block_size;       /* threads per block of the per-particle kernels;
-                               0: 256                                        */
-  int32_t reserved[4];
+  int32_t block_size;       /* threads per block of the per-particle and
+                               per-entry kernels: 64, 128 or 256 (0: 256)    */
+  int32_t narrow_blocks_per_cu; /* resident blocks per CU of the persistent
+                               narrow-phase launch (0: 8)                   */
+  int32_t reserved[3];
 } grains_config_t;
 
 /* Counters of the last grains_step call (summed over its steps). */
@@ -185,6 +187,12 @@ int grains_get_state(grains_ctx_t* ctx, double* pos, double* quat, double* vel,
  * n x 12 Transform3 buffers. */
 int grains_set_transforms(grains_ctx_t* ctx, const double* transforms, uint64_t n);
 int grains_get_transforms(grains_ctx_t* ctx, double* transforms, uint64_t n);
+
+/* Launch shape of the per-particle / per-entry kernels (block_size 64, 128
+ * or 256; narrow_blocks_per_cu >= 1), changeable between steps; used by the
+ * autotuner.  No counterpart in the reference (its kernels hard-code 256,
+ * ComponentManagerGPU.cpp:265-352). */
+int grains_set_launch(grains_ctx_t* ctx, int block_size, int narrow_blocks_per_cu);
 
 /* Time stepping ------------------------------------------------------------ */
 /* n_steps iterations of the GrainsCPU::simulate loop body
