@@ -1,6 +1,7 @@
 // gr_geom.hpp -- narrow phase of the DEM step on the device: support maps,
-// oriented-bounding-box test, GJK distance with Johnson's sub-algorithm,
-// contact dispatch and the Hooke spring-dashpot / Coulomb force.
+// oriented-bounding-box test, GJK distance (Johnson's sub-algorithm for the
+// reference semantics, signed volumes for the physics semantics), contact
+// dispatch and the Hooke spring-dashpot / Coulomb force.
 //
 // Everything is templated on the arithmetic type T (double for the plain
 // path, float for the FP32-mixed path).  The Johnson sub-algorithm keeps its
@@ -543,6 +544,284 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   return out;
 }
 
+// ---- signed-volume distance sub-algorithm (physics semantics) -------------------
+// The closest point of a simplex of up to four points to the origin, from the
+// signs of the origin's barycentric coordinates (ratios of signed volumes and
+// areas), after Montanari, Petrinic and Barbieri (ACM TOG 2017).  No absolute
+// thresholds: a degenerate feature (zero area or volume) hands over to its
+// sub-features, so FP32 and mm-sized grains in SI units need no special care.
+// The simplex is kept newest first (y[0] = last support point) and only
+// features containing y[0] are candidates.  Every index is a template
+// constant; compaction is a fixed network of selects, so the simplex
+// (y, p, q, lambda: 40 values) lives in registers.
+template <class T>
+struct SvSimplex {
+  V3<T> y[4], p[4], q[4];
+  T lam[4];
+};
+template <class T>
+struct SvRes {
+  V3<T> v;
+  T lam[4];
+  unsigned mask;
+};
+template <class T> GR_D SvRes<T> sv_point(V3<T> v) {
+  SvRes<T> r;
+  r.v = v;
+  r.lam[0] = r.lam[1] = r.lam[2] = r.lam[3] = T(0);
+  r.mask = 0;
+  return r;
+}
+template <class T> GR_D SvRes<T> sv_better(const SvRes<T>& a, const SvRes<T>& b) {
+  return norm2(b.v) < norm2(a.v) ? b : a;
+}
+
+template <class T, int IA, int IB>
+GR_D SvRes<T> sv_seg(const SvSimplex<T>& S) {
+  const V3<T> a = S.y[IA], b = S.y[IB];
+  const V3<T> t = b - a;
+  const T tt = dot(t, t);
+  if (tt > T(0)) {
+    const T l = -dot(a, t) / tt;
+    if (l > T(0) && l < T(1)) {
+      SvRes<T> r = sv_point<T>((T(1) - l) * a + l * b);
+      r.lam[IA] = T(1) - l;
+      r.lam[IB] = l;
+      r.mask = (1u << IA) | (1u << IB);
+      return r;
+    }
+    if (l >= T(1)) {
+      SvRes<T> r = sv_point<T>(b);
+      r.lam[IB] = T(1);
+      r.mask = 1u << IB;
+      return r;
+    }
+  }
+  SvRes<T> r = sv_point<T>(a);
+  r.lam[IA] = T(1);
+  r.mask = 1u << IA;
+  return r;
+}
+
+template <class T, int IA, int IB, int IC>
+GR_D SvRes<T> sv_tri(const SvSimplex<T>& S) {
+  const V3<T> a = S.y[IA], b = S.y[IB], c = S.y[IC];
+  const V3<T> n = cross(b - a, c - a);
+  const T nn = dot(n, n);
+  T la = T(0), lb = T(0), lc = T(0);
+  if (nn > T(0)) {
+    la = dot(n, cross(b, c)) / nn;
+    lb = dot(n, cross(c, a)) / nn;
+    lc = dot(n, cross(a, b)) / nn;
+    if (la > T(0) && lb > T(0) && lc > T(0)) {
+      SvRes<T> r = sv_point<T>((la * a + lb * b) + lc * c);
+      r.lam[IA] = la;
+      r.lam[IB] = lb;
+      r.lam[IC] = lc;
+      r.mask = (1u << IA) | (1u << IB) | (1u << IC);
+      return r;
+    }
+  }
+  bool ab = !(nn > T(0)) || lc <= T(0), ac = !(nn > T(0)) || lb <= T(0);
+  if (!ab && !ac) ab = ac = true;  // only la <= 0 (rounding): keep a's edges
+  SvRes<T> best;
+  if (ab) best = sv_seg<T, IA, IB>(S);
+  if (ac) {
+    const SvRes<T> r = sv_seg<T, IA, IC>(S);
+    best = ab ? sv_better(best, r) : r;
+  }
+  return best;
+}
+
+template <class T>
+GR_D SvRes<T> sv_tet(const SvSimplex<T>& S) {
+  const V3<T> a = S.y[0], b = S.y[1], c = S.y[2], d = S.y[3];
+  const V3<T> ab = b - a, ac = c - a, ad = d - a;
+  const T D = dot(ab, cross(ac, ad));
+  T l1 = T(0), l2 = T(0), l3 = T(0);
+  if (D != T(0)) {
+    const T l0 = dot(b, cross(c, d)) / D;
+    l1 = -dot(a, cross(ac, ad)) / D;
+    l2 = dot(ab, cross(-a, ad)) / D;
+    l3 = dot(ab, cross(ac, -a)) / D;
+    if (l0 > T(0) && l1 > T(0) && l2 > T(0) && l3 > T(0)) {
+      SvRes<T> r = sv_point<T>({T(0), T(0), T(0)});
+      r.lam[0] = l0;
+      r.lam[1] = l1;
+      r.lam[2] = l2;
+      r.lam[3] = l3;
+      r.mask = 15u;
+      return r;
+    }
+  }
+  bool fb = D == T(0) || l1 <= T(0), fc = D == T(0) || l2 <= T(0), fd = D == T(0) || l3 <= T(0);
+  if (!fb && !fc && !fd) fb = fc = fd = true;
+  SvRes<T> best;
+  if (fb) best = sv_tri<T, 0, 2, 3>(S);
+  if (fc) {
+    const SvRes<T> r = sv_tri<T, 0, 1, 3>(S);
+    best = fb ? sv_better(best, r) : r;
+  }
+  if (fd) {
+    const SvRes<T> r = sv_tri<T, 0, 1, 2>(S);
+    best = (fb || fc) ? sv_better(best, r) : r;
+  }
+  return best;
+}
+
+// keep the points of r.mask, newest first, with their weights: output slot k
+// takes the (k+1)-th kept input slot i >= k
+template <class T>
+GR_D void sv_compact(SvSimplex<T>& S, int& n, const SvRes<T>& r) {
+  SvSimplex<T> N = S;
+  int c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool on = (r.mask >> i) & 1u;
+#pragma unroll
+    for (int k = 0; k <= i; ++k)
+      if (on && c == k) {
+        N.y[k] = S.y[i];
+        N.p[k] = S.p[i];
+        N.q[k] = S.q[i];
+        N.lam[k] = r.lam[i];
+      }
+    c += on ? 1 : 0;
+  }
+  S = N;
+  n = c;
+}
+
+// GJK with the signed-volume sub-algorithm, physics semantics: crust cores
+// (chosen by the caller's support contexts), the pair's unit frame (power of
+// two sc with rsum * sc in [0.5, 1)), v0 = centre difference, the
+// van den Bergen gap test, NESTEROV momentum on the support direction.
+// `backup` reports a search that ended on rounding (a repeated support point
+// or no decrease of |v|) or on the iteration cap instead of the gap test.
+template <class T>
+GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
+                      const GjkArgs& args, T rsum) {
+  const bool nesterov = (args.flags & GRAINS_NESTEROV) != 0;
+  const T rel = args.rel > 0 ? T(args.rel) : Eps<T>::LOW;
+  Tr<T> b2a = relative(a2w, b2w);
+  int ex;
+  frexp(rsum, &ex);
+  const T sc = ldexp(T(1), -ex);
+  b2a.o = sc * b2a.o;
+  const V3<T> zero = {T(0), T(0), T(0)};
+  SvSimplex<T> S;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    S.y[i] = S.p[i] = S.q[i] = zero;
+    S.lam[i] = T(0);
+  }
+  int n = 0;
+  V3<T> v = -b2a.o, w = zero, pl = zero, ql = zero;
+  T dist = norm(v), mu = T(0);
+  V3<T> dir = v, wprev = v;
+  bool accel = nesterov, inside = false, ran = false, stalled = false;
+  int num_iter = 0;
+  while (dist > Eps<T>::HIGH && num_iter < 1000) {
+    V3<T> d = v;
+    if (accel) {
+      const T delta = T(num_iter + 1) / T(num_iter + 3);
+      const V3<T> yk = delta * v + (T(1) - delta) * wprev;
+      const T ny = norm(yk), nd = norm(dir);
+      if (num_iter != 0 && ny > Eps<T>::HIGH && nd > Eps<T>::HIGH)
+        d = (delta / nd) * dir + ((T(1) - delta) / ny) * yk;
+      dir = d;
+    }
+    pl = sc * support(A, -d);
+    ql = sc * support(B, vm(d, b2a.R));
+    w = pl - apply(b2a, ql);
+    ran = true;
+    const T lb = accel ? dot(d, w) / norm(d) : dot(v, w) / dist;
+    if (mu < lb) mu = lb;
+    if (dist - mu <= dist * rel) break;
+    bool degen = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < n && same(S.y[i], w)) degen = true;
+    if (degen) {
+      if (accel) {
+        accel = false;
+        continue;
+      }
+      stalled = true;
+      break;
+    }
+#pragma unroll
+    for (int i = 3; i > 0; --i) {
+      S.y[i] = S.y[i - 1];
+      S.p[i] = S.p[i - 1];
+      S.q[i] = S.q[i - 1];
+    }
+    S.y[0] = w;
+    S.p[0] = pl;
+    S.q[0] = ql;
+    n = n < 4 ? n + 1 : 4;
+    ++num_iter;
+    SvRes<T> r;
+    if (n == 1) {
+      r = sv_point<T>(w);
+      r.lam[0] = T(1);
+      r.mask = 1u;
+    } else if (n == 2) {
+      r = sv_seg<T, 0, 1>(S);
+    } else if (n == 3) {
+      r = sv_tri<T, 0, 1, 2>(S);
+    } else {
+      r = sv_tet<T>(S);
+    }
+    sv_compact(S, n, r);
+    if (r.mask == 15u) {
+      inside = true;
+      dist = T(0);
+      break;
+    }
+    const T nd = norm(r.v);
+    wprev = w;
+    // no progress: rounding has the last word.  The first support point is
+    // exempt: v0 (the centre difference) is not a point of the simplex.
+    if (num_iter > 1 && !(nd < dist)) {
+      v = r.v;
+      dist = nd;
+      if (accel) {
+        accel = false;
+        continue;
+      }
+      stalled = true;
+      break;
+    }
+    v = r.v;
+    dist = nd;
+  }
+  if (num_iter >= 1000) stalled = true;
+  GjkOut<T> out;
+  if (n == 0) {
+    out.pa = ran ? pl : zero;
+    out.pb = ran ? ql : zero;
+  } else {
+    V3<T> p = zero, q = zero;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < n) {
+        p = p + S.lam[i] * S.p[i];
+        q = q + S.lam[i] * S.q[i];
+      }
+    out.pa = p;
+    out.pb = q;
+  }
+  const T inv = T(1) / sc;
+  out.pa = inv * out.pa;
+  out.pb = inv * out.pb;
+  out.dist = inv * dist;
+  out.iters = num_iter;
+  out.deep = inside || dist <= Eps<T>::HIGH;
+  out.backup = stalled;
+  return out;
+}
+
 // ---- contact information -------------------------------------------------------
 template <class T>
 struct CInfo {
@@ -586,11 +865,13 @@ struct NpStats {
 
 // closestPointsRigidBodies (CollisionDetection.cpp:181-284).  Physics
 // semantics run GJK on the crust cores (the commented intent of lines 231-236).
-template <class T>
+// SV selects the physics semantics (signed-volume GJK on the crust cores);
+// !SV is REF_QUIRKS (Johnson's sub-algorithm on the full shapes).
+template <class T, bool SV>
 GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<T>& a2w,
                              const Tr<T>& b2w, const GjkArgs& args, const T* verts_full,
                              const T* verts_core, NpStats& st) {
-  const bool quirks = (args.flags & GRAINS_REF_QUIRKS) != 0;
+  constexpr bool quirks = !SV;
   if (A.kind == GRAINS_SPHERE && B.kind == GRAINS_SPHERE) return cp_spheres(A.rc, B.rc, a2w, b2w);
   if (B.kind == GRAINS_RECTANGLE) return cp_rectangle(sup_ctx(A, false, verts_full, verts_core), a2w, b2w);
   T overlap = norm(b2w.o - a2w.o) - (A.rc + B.rc);
@@ -598,7 +879,11 @@ GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<
     if (obb_intersect(A.bbox, B.bbox, a2w, b2w)) {
       SupCtx<T> ca = sup_ctx(A, !quirks, verts_full, verts_core);
       SupCtx<T> cb = sup_ctx(B, !quirks, verts_full, verts_core);
-      GjkOut<T> g = gjk_jh(ca, cb, a2w, b2w, args, A.rc + B.rc);
+      GjkOut<T> g;
+      if constexpr (SV)
+        g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc);
+      else
+        g = gjk_jh(ca, cb, a2w, b2w, args, A.rc + B.rc);
       st.gjk = 1;
       st.iters = g.iters;
       st.backup = g.backup;

@@ -44,7 +44,7 @@ __device__ __forceinline__ void pair_frames(const SimGrid& g, V3<double> xi, Qt<
 
 // Narrow phase + Hooke force of entry (i, j) from i's side
 // (ComponentManagerCPU.cpp:384-419).  Returns the pair flag.
-template <class T, class R>
+template <class T, class R, bool SV>
 __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t i, uint32_t j, PairRes<R>& out,
                                                NpStats& st) {
   const uint64_t n = d.n;
@@ -56,7 +56,7 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t i, ui
   Tr<T> a2w, b2w;
   pair_frames<T>(g, xi, load_quat(d.quat, n, i), load3(d.pos, n, j), load_quat(d.quat, n, j), a2w, b2w);
   const GjkArgs args = {d.flags, d.gjk_rel};
-  CInfo<T> ci = closest_points<T>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
+  CInfo<T> ci = closest_points<T, SV>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
   uint32_t flag = (st.gjk ? kFlagGjk : 0u) | (st.deep ? kFlagDeep : 0u);
   if (ci.overlap < T(0)) {
     flag |= kFlagContact;
@@ -92,7 +92,8 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t i, ui
 }
 
 // One lane per candidate entry, grid-stride over the E entries of this step.
-template <class T>
+// SV: physics semantics (signed-volume GJK); !SV: REF_QUIRKS.
+template <class T, bool SV>
 __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   const uint32_t E = d.entry_off[d.n];
   PairRes<T>* res = (PairRes<T>*)d.pair_res;
@@ -101,11 +102,12 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
     NpStats st = {0, 0, 0, 0};
     PairRes<T> r;
-    uint32_t flag = pair_force<T, T>(d, d.pair_i[e], d.pair_j[e], r, st);
+    uint32_t flag = pair_force<T, T, SV>(d, d.pair_i[e], d.pair_j[e], r, st);
     if (flag & kFlagContact) res[e] = r;
     if constexpr (sizeof(T) == 4) {
-      // FP32-mixed: re-run in FP64 when the sub-algorithm fell back, the
-      // cores overlap, or the FP32 result cannot decide contact
+      // FP32-mixed: re-run in FP64 when the search ended on rounding
+      // instead of the gap test, or the cores overlap (the contact normal
+      // switches to the centre line there)
       bool refine = st.backup || st.deep;
       if (refine) {
         flag |= kFlagRefine;
@@ -128,6 +130,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
 }
 
 // FP64 re-run of the entries queued by k_narrow<float>; results stored in FP32.
+template <bool SV>
 __global__ void __launch_bounds__(kBlock) k_refine64(DevScene d);
 
 // Per particle: sum its entries in visit order, walls, gravity, then
@@ -264,7 +267,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
 }
 
 // GrainsTestDev workload: GJK distance of independent pairs of full shapes.
-template <class T>
+template <class T, bool SV>
 __global__ void __launch_bounds__(kBlock) k_gjk_batch(const ShapeNP<T>* __restrict__ shapes,
                                                      const T* __restrict__ verts, const uint32_t* __restrict__ sa,
                                                      const uint32_t* __restrict__ sb, const double* __restrict__ ta,
@@ -292,7 +295,11 @@ __global__ void __launch_bounds__(kBlock) k_gjk_batch(const ShapeNP<T>* __restri
   const ShapeNP<T> A = shapes[sa[k]], B = shapes[sb[k]];
   SupCtx<T> ca = sup_ctx(A, false, verts, verts);
   SupCtx<T> cb = sup_ctx(B, false, verts, verts);
-  GjkOut<T> g = gjk_jh(ca, cb, a2w, b2w, args, A.rc + B.rc);
+  GjkOut<T> g;
+  if constexpr (SV)
+    g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc);
+  else
+    g = gjk_jh(ca, cb, a2w, b2w, args, A.rc + B.rc);
   dist[k] = (double)g.dist;
   if (iters) iters[k] = g.iters;
   if (pa) {

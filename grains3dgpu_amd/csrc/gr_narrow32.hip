@@ -5,7 +5,10 @@
 namespace gr {
 
 void launch_narrow32(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_narrow<float>, dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+  if (d.flags & GRAINS_REF_QUIRKS)
+    hipLaunchKernelGGL((k_narrow<float, false>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+  else
+    hipLaunchKernelGGL((k_narrow<float, true>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
 }
 void launch_integrate32(const DevScene& d, hipStream_t s) {
   hipLaunchKernelGGL(k_integrate<float>, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d);
@@ -13,8 +16,13 @@ void launch_integrate32(const DevScene& d, hipStream_t s) {
 void launch_gjk_batch32(const ShapeNP<float>* shapes, const float* verts, const uint32_t* sa,
                         const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,
                         double* dist, double* pa, double* pb, int32_t* iters, hipStream_t s) {
-  hipLaunchKernelGGL(k_gjk_batch<float>, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                     shapes, verts, sa, sb, ta, tb, n, args, dist, pa, pb, iters);
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+  if (args.flags & GRAINS_REF_QUIRKS)
+    hipLaunchKernelGGL((k_gjk_batch<float, false>), grid, dim3(kBlock), 0, s, shapes, verts, sa, sb, ta, tb, n,
+                       args, dist, pa, pb, iters);
+  else
+    hipLaunchKernelGGL((k_gjk_batch<float, true>), grid, dim3(kBlock), 0, s, shapes, verts, sa, sb, ta, tb, n,
+                       args, dist, pa, pb, iters);
 }
 
 }  // namespace gr

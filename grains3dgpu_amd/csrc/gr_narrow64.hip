@@ -6,6 +6,7 @@
 
 namespace gr {
 
+template <bool SV>
 __global__ void __launch_bounds__(kBlock) k_refine64(DevScene d) {
   const uint32_t cnt = *d.refine_count;
   PairRes<float>* res = (PairRes<float>*)d.pair_res;
@@ -14,17 +15,23 @@ __global__ void __launch_bounds__(kBlock) k_refine64(DevScene d) {
     const uint32_t e = d.refine_list[k];
     NpStats st = {0, 0, 0, 0};
     PairRes<float> r;
-    uint32_t flag = pair_force<double, float>(d, d.pair_i[e], d.pair_j[e], r, st);
+    uint32_t flag = pair_force<double, float, SV>(d, d.pair_i[e], d.pair_j[e], r, st);
     if (flag & kFlagContact) res[e] = r;
     d.pair_flag[e] = flag | kFlagRefine;
   }
 }
 
 void launch_narrow64(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_narrow<double>, dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+  if (d.flags & GRAINS_REF_QUIRKS)
+    hipLaunchKernelGGL((k_narrow<double, false>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+  else
+    hipLaunchKernelGGL((k_narrow<double, true>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
 }
 void launch_refine64(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_refine64, dim3(d.narrow_blocks / 4 + 1), dim3(kBlock), 0, s, d);
+  if (d.flags & GRAINS_REF_QUIRKS)
+    hipLaunchKernelGGL(k_refine64<false>, dim3(d.narrow_blocks / 4 + 1), dim3(kBlock), 0, s, d);
+  else
+    hipLaunchKernelGGL(k_refine64<true>, dim3(d.narrow_blocks / 4 + 1), dim3(kBlock), 0, s, d);
 }
 void launch_integrate64(const DevScene& d, hipStream_t s) {
   hipLaunchKernelGGL(k_integrate<double>, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d);
@@ -32,8 +39,13 @@ void launch_integrate64(const DevScene& d, hipStream_t s) {
 void launch_gjk_batch64(const ShapeNP<double>* shapes, const double* verts, const uint32_t* sa,
                         const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,
                         double* dist, double* pa, double* pb, int32_t* iters, hipStream_t s) {
-  hipLaunchKernelGGL(k_gjk_batch<double>, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                     shapes, verts, sa, sb, ta, tb, n, args, dist, pa, pb, iters);
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+  if (args.flags & GRAINS_REF_QUIRKS)
+    hipLaunchKernelGGL((k_gjk_batch<double, false>), grid, dim3(kBlock), 0, s, shapes, verts, sa, sb, ta, tb, n,
+                       args, dist, pa, pb, iters);
+  else
+    hipLaunchKernelGGL((k_gjk_batch<double, true>), grid, dim3(kBlock), 0, s, shapes, verts, sa, sb, ta, tb, n,
+                       args, dist, pa, pb, iters);
 }
 
 }  // namespace gr

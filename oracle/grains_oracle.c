@@ -709,6 +709,217 @@ static int jh_degenerate(unsigned all_bits, const v3* y, v3 w) {
   return 0;
 }
 
+/* ---- signed-volume distance sub-algorithm (physics semantics) -------------
+ * The closest point of a simplex of up to four points to the origin, from
+ * the signs of the origin's barycentric coordinates (ratios of signed
+ * volumes / areas), after the method of Montanari, Petrinic and Barbieri
+ * (ACM TOG 2017), of which the reference's unused GJK_SV.cpp is another
+ * rendition.  No absolute thresholds: a degenerate feature (zero area or
+ * volume) hands over to its sub-features.  y[0] is the newest point; only
+ * features containing it are candidates, as GJK guarantees. */
+typedef struct {
+  v3 y[4], p[4], q[4];
+  double lam[4];
+  int n;
+} sv_simplex;
+typedef struct {
+  v3 v;
+  double lam[4];
+  unsigned mask;
+} sv_res;
+
+static sv_res sv_seg(const sv_simplex* S, int ia, int ib) {
+  sv_res r;
+  memset(&r, 0, sizeof(r));
+  v3 a = S->y[ia], b = S->y[ib];
+  v3 t = vsub(b, a);
+  double tt = vdot(t, t);
+  if (tt > 0.0) {
+    double l = -vdot(a, t) / tt;
+    if (l > 0.0 && l < 1.0) {
+      r.v = vadd(vscale(1.0 - l, a), vscale(l, b));
+      r.lam[ia] = 1.0 - l;
+      r.lam[ib] = l;
+      r.mask = (1u << ia) | (1u << ib);
+      return r;
+    }
+    if (l >= 1.0) {
+      r.v = b;
+      r.lam[ib] = 1.0;
+      r.mask = 1u << ib;
+      return r;
+    }
+  }
+  r.v = a;
+  r.lam[ia] = 1.0;
+  r.mask = 1u << ia;
+  return r;
+}
+
+static sv_res sv_better(sv_res a, sv_res b) { return vnorm2(b.v) < vnorm2(a.v) ? b : a; }
+
+static sv_res sv_tri(const sv_simplex* S, int ia, int ib, int ic) {
+  v3 a = S->y[ia], b = S->y[ib], c = S->y[ic];
+  v3 n = vcross(vsub(b, a), vsub(c, a));
+  double nn = vdot(n, n);
+  double la = 0.0, lb = 0.0, lc = 0.0;
+  if (nn > 0.0) {
+    la = vdot(n, vcross(b, c)) / nn;
+    lb = vdot(n, vcross(c, a)) / nn;
+    lc = vdot(n, vcross(a, b)) / nn;
+    if (la > 0.0 && lb > 0.0 && lc > 0.0) {
+      sv_res r;
+      memset(&r, 0, sizeof(r));
+      r.v = vadd(vadd(vscale(la, a), vscale(lb, b)), vscale(lc, c));
+      r.lam[ia] = la;
+      r.lam[ib] = lb;
+      r.lam[ic] = lc;
+      r.mask = (1u << ia) | (1u << ib) | (1u << ic);
+      return r;
+    }
+  }
+  int ab = !(nn > 0.0) || lc <= 0.0, ac = !(nn > 0.0) || lb <= 0.0;
+  if (!ab && !ac) ab = ac = 1; /* only la <= 0: rounding; keep a's edges */
+  if (ab && ac) return sv_better(sv_seg(S, ia, ib), sv_seg(S, ia, ic));
+  return ab ? sv_seg(S, ia, ib) : sv_seg(S, ia, ic);
+}
+
+static sv_res sv_tet(const sv_simplex* S) {
+  v3 a = S->y[0], b = S->y[1], c = S->y[2], d = S->y[3];
+  v3 ab = vsub(b, a), ac = vsub(c, a), ad = vsub(d, a);
+  double D = vdot(ab, vcross(ac, ad));
+  double l[4] = {0, 0, 0, 0};
+  if (D != 0.0) {
+    l[0] = vdot(b, vcross(c, d)) / D;
+    l[1] = -vdot(a, vcross(ac, ad)) / D;
+    l[2] = vdot(ab, vcross(vneg(a), ad)) / D;
+    l[3] = vdot(ab, vcross(ac, vneg(a))) / D;
+    if (l[0] > 0.0 && l[1] > 0.0 && l[2] > 0.0 && l[3] > 0.0) {
+      sv_res r;
+      memset(&r, 0, sizeof(r));
+      r.v = V(0.0, 0.0, 0.0);
+      for (int i = 0; i < 4; ++i) r.lam[i] = l[i];
+      r.mask = 15u;
+      return r;
+    }
+  }
+  int fb = D == 0.0 || l[1] <= 0.0, fc = D == 0.0 || l[2] <= 0.0, fd = D == 0.0 || l[3] <= 0.0;
+  if (!fb && !fc && !fd) fb = fc = fd = 1;
+  sv_res best;
+  int have = 0;
+  if (fb) { best = sv_tri(S, 0, 2, 3); have = 1; }
+  if (fc) { sv_res r = sv_tri(S, 0, 1, 3); best = have ? sv_better(best, r) : r; have = 1; }
+  if (fd) { sv_res r = sv_tri(S, 0, 1, 2); best = have ? sv_better(best, r) : r; }
+  return best;
+}
+
+/* keep the points of r.mask, newest first, with their weights */
+static void sv_compact(sv_simplex* S, const sv_res* r) {
+  sv_simplex N;
+  memset(&N, 0, sizeof(N));
+  for (int i = 0; i < 4; ++i)
+    if (r->mask & (1u << i)) {
+      N.y[N.n] = S->y[i];
+      N.p[N.n] = S->p[i];
+      N.q[N.n] = S->q[i];
+      N.lam[N.n] = r->lam[i];
+      ++N.n;
+    }
+  *S = N;
+}
+
+/* GJK with the signed-volume sub-algorithm, physics semantics: crust cores,
+ * the pair's unit frame (power-of-two scale), v0 = centre difference, the
+ * van den Bergen gap test, NESTEROV momentum on the support direction. */
+static double gjk_sv(const oshape* A, const oshape* B, const tr3* a2w, const tr3* b2w,
+                     uint32_t flags, double rel_tol, v3* pa, v3* pb, int* nb_iter, int* deep) {
+  const int nesterov = (flags & GRAINS_NESTEROV) != 0;
+  const double rel = rel_tol > 0 ? rel_tol : LOWEPS;
+  tr3 b2a = trelative(a2w, b2w);
+  int e;
+  frexp(A->rc_exact + B->rc_exact, &e);
+  const double sc = ldexp(1.0, -e);
+  b2a.o = vscale(sc, b2a.o);
+  sv_simplex S;
+  memset(&S, 0, sizeof(S));
+  v3 v = vneg(b2a.o), w = V(0, 0, 0), pl = V(0, 0, 0), ql = V(0, 0, 0);
+  double dist = vnorm(v), mu = 0.0;
+  v3 dir = v, wprev = v;
+  int accel = nesterov, num_iter = 0, inside = 0, ran = 0;
+  while (dist > HIGHEPS && num_iter < 1000) {
+    v3 d = v;
+    if (accel) {
+      double delta = (double)(num_iter + 1) / (double)(num_iter + 3);
+      v3 yk = vadd(vscale(delta, v), vscale(1.0 - delta, wprev));
+      double ny = vnorm(yk), nd = vnorm(dir);
+      if (num_iter != 0 && ny > HIGHEPS && nd > HIGHEPS)
+        d = vadd(vscale(delta / nd, dir), vscale((1.0 - delta) / ny, yk));
+      dir = d;
+    }
+    pl = vscale(sc, support(A, vneg(d), 1));
+    ql = vscale(sc, support(B, vm(d, &b2a.R), 1));
+    w = vsub(pl, tapply(&b2a, ql));
+    ran = 1;
+    double lb = accel ? vdot(d, w) / vnorm(d) : vdot(v, w) / dist;
+    if (mu < lb) mu = lb;
+    if (dist - mu <= dist * rel) break;
+    int degen = 0;
+    for (int i = 0; i < S.n; ++i)
+      if (S.y[i].x == w.x && S.y[i].y == w.y && S.y[i].z == w.z) degen = 1;
+    if (degen) {
+      if (accel) { accel = 0; continue; }
+      break;
+    }
+    for (int i = 3; i > 0; --i) { S.y[i] = S.y[i - 1]; S.p[i] = S.p[i - 1]; S.q[i] = S.q[i - 1]; }
+    S.y[0] = w;
+    S.p[0] = pl;
+    S.q[0] = ql;
+    S.n = S.n < 4 ? S.n + 1 : 4;
+    ++num_iter;
+    sv_res r;
+    if (S.n == 1) {
+      memset(&r, 0, sizeof(r));
+      r.v = w;
+      r.lam[0] = 1.0;
+      r.mask = 1u;
+    } else if (S.n == 2) {
+      r = sv_seg(&S, 0, 1);
+    } else if (S.n == 3) {
+      r = sv_tri(&S, 0, 1, 2);
+    } else {
+      r = sv_tet(&S);
+    }
+    sv_compact(&S, &r);
+    if (r.mask == 15u) { inside = 1; dist = 0.0; break; }
+    double nd = vnorm(r.v);
+    wprev = w;
+    /* no progress: rounding has the last word.  The first support point is
+     * exempt: v0 (the centre difference) is not a point of the simplex. */
+    if (num_iter > 1 && !(nd < dist)) {
+      if (accel) { accel = 0; v = r.v; dist = nd; continue; }
+      v = r.v;
+      dist = nd;
+      break;
+    }
+    v = r.v;
+    dist = nd;
+  }
+  v3 p = V(0, 0, 0), q = V(0, 0, 0);
+  if (S.n == 0) {
+    if (ran) { p = pl; q = ql; }
+  } else {
+    for (int i = 0; i < S.n; ++i) {
+      p = vadd(p, vscale(S.lam[i], S.p[i]));
+      q = vadd(q, vscale(S.lam[i], S.q[i]));
+    }
+  }
+  *pa = vscale(1.0 / sc, p);
+  *pb = vscale(1.0 / sc, q);
+  if (deep) *deep = inside || dist <= HIGHEPS;
+  *nb_iter = num_iter;
+  return dist / sc;
+}
+
 /* computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391).
  * REF_QUIRKS: exactly the reference, incl. v0 = s_A(0) - b2a(s_B(0)) and the
  * break on mu < 1e-4*relError.  Physics: v0 = centre difference, no absolute
@@ -879,7 +1090,9 @@ static cinfo closest_points(const oshape* A, const oshape* B, const tr3* a2w, co
     if (obb_intersect(A->bbox, B->bbox, a2w, b2w)) {
       v3 ptA, ptB;
       int it = 0, deep = 0;
-      double distance = gjk_jh(A, B, a2w, b2w, flags, rel_tol, &ptA, &ptB, &it, &deep);
+      double distance = (flags & GRAINS_REF_QUIRKS)
+                            ? gjk_jh(A, B, a2w, b2w, flags, rel_tol, &ptA, &ptB, &it, &deep)
+                            : gjk_sv(A, B, a2w, b2w, flags, rel_tol, &ptA, &ptB, &it, &deep);
       if (cnt) {
         cnt->gjk_calls++;
         cnt->gjk_iters += (uint64_t)it;
@@ -1591,7 +1804,8 @@ double go_gjk(const grains_shape_t* a, const grains_shape_t* b, const double* ve
   tr3 TA = tr_from12(ta), TB = tr_from12(tb);
   v3 p, q;
   int it = 0;
-  double d = gjk_jh(&A, &B, &TA, &TB, flags, rel_tol, &p, &q, &it, NULL);
+  double d = (flags & GRAINS_REF_QUIRKS) ? gjk_jh(&A, &B, &TA, &TB, flags, rel_tol, &p, &q, &it, NULL)
+                                         : gjk_sv(&A, &B, &TA, &TB, flags, rel_tol, &p, &q, &it, NULL);
   p = tapply(&TA, p);
   q = tapply(&TB, q);
   if (pa) { pa[0] = p.x; pa[1] = p.y; pa[2] = p.z; }

