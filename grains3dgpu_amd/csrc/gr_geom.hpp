@@ -396,24 +396,17 @@ struct GjkOut {
   V3<T> pa, pb;   // witness points in the local frames of A and B
   int iters;
   bool deep;      // physics semantics: crust cores overlap
-  bool backup;    // the sub-algorithm fell back to its backup procedure
+  bool backup;    // Johnson: the backup procedure was taken; signed volumes:
+                  // the search ended on rounding or the cap, not the gap test
 };
 
-// computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391).  REF_QUIRKS: the
-// reference exactly, incl. v0 = s_A(0) - b2a(s_B(0)) and the break on
-// mu < 1e-4*rel.  Physics: v0 = centre difference, the standard
-// van den Bergen termination, |v| of the backup simplex, witnesses of the
-// last support pair when no simplex was built; NESTEROV adds momentum on the
-// support direction (Montaut et al. 2022) with the lower bound d.w/|d|.
-// `rsum` = R_A + R_B: with physics semantics every support point and the
-// offset of B are multiplied by the power of two sc, (rsum * sc) in [0.5, 1),
-// so the sub-algorithm's absolute thresholds (det <= EPS) see determinants of
-// order one whatever the unit of length; a power of two scales exactly.
+// computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391), exactly the reference
+// (the REF_QUIRKS semantics): full shapes, v0 = s_A(0) - b2a(s_B(0)), the
+// break on mu < 1e-4*rel, the stale |v| kept when Johnson's backup procedure
+// is taken.  The physics semantics use gjk_sv.
 template <class T>
 GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
-                      const GjkArgs& args, T rsum) {
-  const bool quirks = (args.flags & GRAINS_REF_QUIRKS) != 0;
-  const bool nesterov = !quirks && (args.flags & GRAINS_NESTEROV);
+                      const GjkArgs& args) {
   const T rel = args.rel > 0 ? T(args.rel) : Eps<T>::LOW;
   const T abs_err = T(1.e-4) * rel;
   JH<T> S;
@@ -431,22 +424,11 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   int last = 0;
   T mu = T(0);
   int num_iter = 0;
-  Tr<T> b2a = relative(a2w, b2w);
-  T sc = T(1);
-  if (!quirks) {
-    int ex;
-    frexp(rsum, &ex);
-    sc = ldexp(T(1), -ex);
-    b2a.o = sc * b2a.o;
-  }
+  const Tr<T> b2a = relative(a2w, b2w);
   const V3<T> zero = {T(0), T(0), T(0)};
-  V3<T> v = quirks ? support(A, zero) - apply(b2a, support(B, zero)) : -b2a.o;
-  V3<T> w = zero, pl = zero, ql = zero;
+  V3<T> v = support(A, zero) - apply(b2a, support(B, zero));
   T dist = norm(v);
-  V3<T> dir = v, wprev = v;
-  bool accel = nesterov;
   bool backup = false;
-  bool ran = false;
   while (bits < 15 && dist > Eps<T>::HIGH && num_iter < 1000) {
     last = 0;
     unsigned last_bit = 1;
@@ -454,37 +436,17 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
       ++last;
       last_bit <<= 1;
     }
-    V3<T> d = v;
-    if (accel) {
-      T delta = T(num_iter + 1) / T(num_iter + 3);
-      V3<T> yk = delta * v + (T(1) - delta) * wprev;
-      T ny = norm(yk), nd = norm(dir);
-      if (num_iter != 0 && ny > Eps<T>::HIGH && nd > Eps<T>::HIGH)
-        d = (delta / nd) * dir + ((T(1) - delta) / ny) * yk;
-      dir = d;
-    }
-    pl = sc * support(A, -d);
-    ql = sc * support(B, vm(d, b2a.R));
-    w = pl - apply(b2a, ql);
-    ran = true;
-    T lb = accel ? dot(d, w) / norm(d) : dot(v, w) / dist;
+    const V3<T> pl = support(A, -v);
+    const V3<T> ql = support(B, vm(v, b2a.R));
+    const V3<T> w = pl - apply(b2a, ql);
+    const T lb = dot(v, w) / dist;
     if (mu < lb) mu = lb;
-    if (quirks) {
-      if (dist - mu <= dist * rel || mu < abs_err) break;
-    } else if (dist - mu <= dist * rel) {
-      break;
-    }
+    if (dist - mu <= dist * rel || mu < abs_err) break;
     bool degen = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       if ((all_bits & (1u << i)) && same(S.y[i], w)) degen = true;
-    if (degen) {
-      if (accel) {
-        accel = false;
-        continue;
-      }
-      break;
-    }
+    if (degen) break;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       if (i == last) {
@@ -494,7 +456,6 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
       }
     all_bits = bits | last_bit;
     ++num_iter;
-    const T dist_before = dist;
     bool ok;
     switch (last) {
       case 0: ok = jh_closest<T, 0>(S, bits, all_bits, v); break;
@@ -503,44 +464,22 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
       default: ok = jh_closest<T, 3>(S, bits, all_bits, v); break;
     }
     if (!ok) {
-      // the reference keeps the stale |v| and stops; the physics semantics
-      // use the returned simplex's |v|, and an accelerated search continues
-      // from it as plain GJK
-      if (!quirks) dist = norm(v);
-      if (accel) {
-        accel = false;
-        wprev = w;
-        continue;
-      }
       backup = true;
       break;
     }
     dist = norm(v);
-    wprev = w;
-    if (accel && dist > dist_before * (T(1) - T(1e-12))) accel = false;
   }
   GjkOut<T> out;
-  if (!quirks && bits == 0) {
-    out.pa = ran ? pl : zero;
-    out.pb = ran ? ql : zero;
-  } else {
-    jh_points_dispatch<T>(S, bits, out.pa, out.pb, std::make_integer_sequence<int, 15>{});
-    if (bits == 0) {  // reference: computePoints with an empty simplex gives 0 * (1/0)
-      T s = T(1) / T(0);
-      out.pa = s * zero;
-      out.pb = s * zero;
-    }
+  jh_points_dispatch<T>(S, bits, out.pa, out.pb, std::make_integer_sequence<int, 15>{});
+  if (bits == 0) {  // reference: computePoints with an empty simplex gives 0 * (1/0)
+    const T s = T(1) / T(0);
+    out.pa = s * zero;
+    out.pb = s * zero;
   }
   out.dist = dist;
   out.iters = num_iter;
-  out.deep = !quirks && (bits == 15 || dist <= Eps<T>::HIGH);
+  out.deep = false;
   out.backup = backup;
-  if (!quirks) {  // back to the caller's unit of length
-    const T inv = T(1) / sc;
-    out.pa = inv * out.pa;
-    out.pb = inv * out.pb;
-    out.dist = inv * out.dist;
-  }
   return out;
 }
 
@@ -551,9 +490,15 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
 // thresholds: a degenerate feature (zero area or volume) hands over to its
 // sub-features, so FP32 and mm-sized grains in SI units need no special care.
 // The simplex is kept newest first (y[0] = last support point) and only
-// features containing y[0] are candidates.  Every index is a template
-// constant; compaction is a fixed network of selects, so the simplex
-// (y, p, q, lambda: 40 values) lives in registers.
+// features containing y[0] are candidates.  Every ratio is taken relative to
+// the vertex a = y[0]: edges e = y - a are exact for nearby points and the
+// coordinates come from products of a with edges (n.(e2 x a), a.(ac x ad)),
+// never from cross products of two far-away vertices such as b x c, whose
+// cancellation costs eps |a|^2 / |n| once the simplex has shrunk to the
+// contact (|e| ~ 1e-6 |a| near convergence); the first coordinate is 1 minus
+// the others and the closest point is a + sum lambda_k e_k.  Every index is a
+// template constant; compaction is a fixed network of selects, so the
+// simplex (y, p, q, lambda: 40 values) lives in registers.
 template <class T>
 struct SvSimplex {
   V3<T> y[4], p[4], q[4];
@@ -584,7 +529,7 @@ GR_D SvRes<T> sv_seg(const SvSimplex<T>& S) {
   if (tt > T(0)) {
     const T l = -dot(a, t) / tt;
     if (l > T(0) && l < T(1)) {
-      SvRes<T> r = sv_point<T>((T(1) - l) * a + l * b);
+      SvRes<T> r = sv_point<T>(a + l * t);
       r.lam[IA] = T(1) - l;
       r.lam[IB] = l;
       r.mask = (1u << IA) | (1u << IB);
@@ -605,16 +550,17 @@ GR_D SvRes<T> sv_seg(const SvSimplex<T>& S) {
 
 template <class T, int IA, int IB, int IC>
 GR_D SvRes<T> sv_tri(const SvSimplex<T>& S) {
-  const V3<T> a = S.y[IA], b = S.y[IB], c = S.y[IC];
-  const V3<T> n = cross(b - a, c - a);
+  const V3<T> a = S.y[IA];
+  const V3<T> e1 = S.y[IB] - a, e2 = S.y[IC] - a;
+  const V3<T> n = cross(e1, e2);
   const T nn = dot(n, n);
   T la = T(0), lb = T(0), lc = T(0);
   if (nn > T(0)) {
-    la = dot(n, cross(b, c)) / nn;
-    lb = dot(n, cross(c, a)) / nn;
-    lc = dot(n, cross(a, b)) / nn;
+    lb = dot(n, cross(e2, a)) / nn;
+    lc = dot(n, cross(a, e1)) / nn;
+    la = (T(1) - lb) - lc;
     if (la > T(0) && lb > T(0) && lc > T(0)) {
-      SvRes<T> r = sv_point<T>((la * a + lb * b) + lc * c);
+      SvRes<T> r = sv_point<T>((a + lb * e1) + lc * e2);
       r.lam[IA] = la;
       r.lam[IB] = lb;
       r.lam[IC] = lc;
@@ -635,15 +581,15 @@ GR_D SvRes<T> sv_tri(const SvSimplex<T>& S) {
 
 template <class T>
 GR_D SvRes<T> sv_tet(const SvSimplex<T>& S) {
-  const V3<T> a = S.y[0], b = S.y[1], c = S.y[2], d = S.y[3];
-  const V3<T> ab = b - a, ac = c - a, ad = d - a;
+  const V3<T> a = S.y[0];
+  const V3<T> ab = S.y[1] - a, ac = S.y[2] - a, ad = S.y[3] - a;
   const T D = dot(ab, cross(ac, ad));
   T l1 = T(0), l2 = T(0), l3 = T(0);
   if (D != T(0)) {
-    const T l0 = dot(b, cross(c, d)) / D;
     l1 = -dot(a, cross(ac, ad)) / D;
-    l2 = dot(ab, cross(-a, ad)) / D;
-    l3 = dot(ab, cross(ac, -a)) / D;
+    l2 = dot(ab, cross(ad, a)) / D;
+    l3 = dot(ab, cross(a, ac)) / D;
+    const T l0 = ((T(1) - l1) - l2) - l3;
     if (l0 > T(0) && l1 > T(0) && l2 > T(0) && l3 > T(0)) {
       SvRes<T> r = sv_point<T>({T(0), T(0), T(0)});
       r.lam[0] = l0;
@@ -692,12 +638,19 @@ GR_D void sv_compact(SvSimplex<T>& S, int& n, const SvRes<T>& r) {
   n = c;
 }
 
+// Momentum is applied to this many support points, then plain GJK: on
+// superquadric pairs this takes about 20 % fewer iterations than plain GJK,
+// with a worst case no worse; uncapped, the momentum zig-zags near the
+// optimum and can reach the iteration cap.
+constexpr int kNesterovIters = 8;
+
 // GJK with the signed-volume sub-algorithm, physics semantics: crust cores
 // (chosen by the caller's support contexts), the pair's unit frame (power of
 // two sc with rsum * sc in [0.5, 1)), v0 = centre difference, the
-// van den Bergen gap test, NESTEROV momentum on the support direction.
-// `backup` reports a search that ended on rounding (a repeated support point
-// or no decrease of |v|) or on the iteration cap instead of the gap test.
+// van den Bergen gap test, NESTEROV momentum on the first kNesterovIters
+// support directions (Montaut et al. 2022).  `backup` reports a search that
+// ended on rounding (a repeated support point or no decrease of |v|) or on
+// the iteration cap instead of the gap test.
 template <class T>
 GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
                       const GjkArgs& args, T rsum) {
@@ -723,6 +676,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   int num_iter = 0;
   while (dist > Eps<T>::HIGH && num_iter < 1000) {
     V3<T> d = v;
+    if (accel && num_iter >= kNesterovIters) accel = false;
     if (accel) {
       const T delta = T(num_iter + 1) / T(num_iter + 3);
       const V3<T> yk = delta * v + (T(1) - delta) * wprev;
@@ -750,16 +704,21 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
       stalled = true;
       break;
     }
+    // the simplex holds at most 3 points here (a tetrahedron either encloses
+    // the origin or is reduced), so the shift loses nothing and can be undone
 #pragma unroll
     for (int i = 3; i > 0; --i) {
       S.y[i] = S.y[i - 1];
       S.p[i] = S.p[i - 1];
       S.q[i] = S.q[i - 1];
+      S.lam[i] = S.lam[i - 1];
     }
     S.y[0] = w;
     S.p[0] = pl;
     S.q[0] = ql;
-    n = n < 4 ? n + 1 : 4;
+    S.lam[0] = T(0);
+    const int n_prev = n;
+    n = n + 1;
     ++num_iter;
     SvRes<T> r;
     if (n == 1) {
@@ -773,19 +732,26 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
     } else {
       r = sv_tet<T>(S);
     }
-    sv_compact(S, n, r);
     if (r.mask == 15u) {
+      sv_compact(S, n, r);
       inside = true;
       dist = T(0);
       break;
     }
     const T nd = norm(r.v);
-    wprev = w;
-    // no progress: rounding has the last word.  The first support point is
-    // exempt: v0 (the centre difference) is not a point of the simplex.
+    // No progress: rounding has the last word.  Drop w and restore the
+    // previous simplex, whose |v| is the best bound found; an accelerated
+    // search continues unaccelerated, a plain one stops.  The first support
+    // point is exempt: v0 (the centre difference) is not a simplex point.
     if (num_iter > 1 && !(nd < dist)) {
-      v = r.v;
-      dist = nd;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        S.y[i] = S.y[i + 1];
+        S.p[i] = S.p[i + 1];
+        S.q[i] = S.q[i + 1];
+        S.lam[i] = S.lam[i + 1];
+      }
+      n = n_prev;
       if (accel) {
         accel = false;
         continue;
@@ -793,6 +759,8 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
       stalled = true;
       break;
     }
+    sv_compact(S, n, r);
+    wprev = w;
     v = r.v;
     dist = nd;
   }
@@ -883,7 +851,7 @@ GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<
       if constexpr (SV)
         g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc);
       else
-        g = gjk_jh(ca, cb, a2w, b2w, args, A.rc + B.rc);
+        g = gjk_jh(ca, cb, a2w, b2w, args);
       st.gjk = 1;
       st.iters = g.iters;
       st.backup = g.backup;

@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(kBlock) k_gjk_batch(const ShapeNP<T>* __restri
   if constexpr (SV)
     g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc);
   else
-    g = gjk_jh(ca, cb, a2w, b2w, args, A.rc + B.rc);
+    g = gjk_jh(ca, cb, a2w, b2w, args);
   dist[k] = (double)g.dist;
   if (iters) iters[k] = g.iters;
   if (pa) {

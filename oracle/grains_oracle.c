@@ -716,7 +716,14 @@ static int jh_degenerate(unsigned all_bits, const v3* y, v3 w) {
  * (ACM TOG 2017), of which the reference's unused GJK_SV.cpp is another
  * rendition.  No absolute thresholds: a degenerate feature (zero area or
  * volume) hands over to its sub-features.  y[0] is the newest point; only
- * features containing it are candidates, as GJK guarantees. */
+ * features containing it are candidates, as GJK guarantees.
+ * Every ratio is taken relative to the vertex a = y[0]: edges e = y - a are
+ * exact for nearby points, and the coordinates come from products of a with
+ * edges (n.(e2 x a), a.(ac x ad), ...), never from cross products of two
+ * far-away vertices such as b x c, whose cancellation costs eps |a|^2 / |n|
+ * when the simplex has shrunk to the contact (|e| ~ 1e-6 |a| near
+ * convergence).  The first coordinate is 1 minus the others, and the closest
+ * point is a + sum lambda_k e_k. */
 typedef struct {
   v3 y[4], p[4], q[4];
   double lam[4];
@@ -737,7 +744,7 @@ static sv_res sv_seg(const sv_simplex* S, int ia, int ib) {
   if (tt > 0.0) {
     double l = -vdot(a, t) / tt;
     if (l > 0.0 && l < 1.0) {
-      r.v = vadd(vscale(1.0 - l, a), vscale(l, b));
+      r.v = vadd(a, vscale(l, t));
       r.lam[ia] = 1.0 - l;
       r.lam[ib] = l;
       r.mask = (1u << ia) | (1u << ib);
@@ -759,18 +766,19 @@ static sv_res sv_seg(const sv_simplex* S, int ia, int ib) {
 static sv_res sv_better(sv_res a, sv_res b) { return vnorm2(b.v) < vnorm2(a.v) ? b : a; }
 
 static sv_res sv_tri(const sv_simplex* S, int ia, int ib, int ic) {
-  v3 a = S->y[ia], b = S->y[ib], c = S->y[ic];
-  v3 n = vcross(vsub(b, a), vsub(c, a));
+  v3 a = S->y[ia];
+  v3 e1 = vsub(S->y[ib], a), e2 = vsub(S->y[ic], a);
+  v3 n = vcross(e1, e2);
   double nn = vdot(n, n);
   double la = 0.0, lb = 0.0, lc = 0.0;
   if (nn > 0.0) {
-    la = vdot(n, vcross(b, c)) / nn;
-    lb = vdot(n, vcross(c, a)) / nn;
-    lc = vdot(n, vcross(a, b)) / nn;
+    lb = vdot(n, vcross(e2, a)) / nn;
+    lc = vdot(n, vcross(a, e1)) / nn;
+    la = (1.0 - lb) - lc;
     if (la > 0.0 && lb > 0.0 && lc > 0.0) {
       sv_res r;
       memset(&r, 0, sizeof(r));
-      r.v = vadd(vadd(vscale(la, a), vscale(lb, b)), vscale(lc, c));
+      r.v = vadd(vadd(a, vscale(lb, e1)), vscale(lc, e2));
       r.lam[ia] = la;
       r.lam[ib] = lb;
       r.lam[ic] = lc;
@@ -785,15 +793,15 @@ static sv_res sv_tri(const sv_simplex* S, int ia, int ib, int ic) {
 }
 
 static sv_res sv_tet(const sv_simplex* S) {
-  v3 a = S->y[0], b = S->y[1], c = S->y[2], d = S->y[3];
-  v3 ab = vsub(b, a), ac = vsub(c, a), ad = vsub(d, a);
+  v3 a = S->y[0];
+  v3 ab = vsub(S->y[1], a), ac = vsub(S->y[2], a), ad = vsub(S->y[3], a);
   double D = vdot(ab, vcross(ac, ad));
   double l[4] = {0, 0, 0, 0};
   if (D != 0.0) {
-    l[0] = vdot(b, vcross(c, d)) / D;
     l[1] = -vdot(a, vcross(ac, ad)) / D;
-    l[2] = vdot(ab, vcross(vneg(a), ad)) / D;
-    l[3] = vdot(ab, vcross(ac, vneg(a))) / D;
+    l[2] = vdot(ab, vcross(ad, a)) / D;
+    l[3] = vdot(ab, vcross(a, ac)) / D;
+    l[0] = ((1.0 - l[1]) - l[2]) - l[3];
     if (l[0] > 0.0 && l[1] > 0.0 && l[2] > 0.0 && l[3] > 0.0) {
       sv_res r;
       memset(&r, 0, sizeof(r));
@@ -828,6 +836,13 @@ static void sv_compact(sv_simplex* S, const sv_res* r) {
   *S = N;
 }
 
+/* Momentum is applied to this many support points, then plain GJK: on the
+ * GJK distance test set of superquadric pairs this takes 25 % fewer
+ * iterations than plain GJK at gap 1e-6 (19 % at 1e-10), with a worst case no
+ * worse than plain GJK; uncapped, the momentum zig-zags near the optimum and
+ * can reach the iteration cap. */
+#define SV_NESTEROV_ITERS 8
+
 /* GJK with the signed-volume sub-algorithm, physics semantics: crust cores,
  * the pair's unit frame (power-of-two scale), v0 = centre difference, the
  * van den Bergen gap test, NESTEROV momentum on the support direction. */
@@ -848,6 +863,9 @@ static double gjk_sv(const oshape* A, const oshape* B, const tr3* a2w, const tr3
   int accel = nesterov, num_iter = 0, inside = 0, ran = 0;
   while (dist > HIGHEPS && num_iter < 1000) {
     v3 d = v;
+    /* momentum on the first SV_NESTEROV_ITERS support points only: near the
+     * optimum it zig-zags (sublinear on superquadrics) */
+    if (accel && num_iter >= SV_NESTEROV_ITERS) accel = 0;
     if (accel) {
       double delta = (double)(num_iter + 1) / (double)(num_iter + 3);
       v3 yk = vadd(vscale(delta, v), vscale(1.0 - delta, wprev));
@@ -870,11 +888,21 @@ static double gjk_sv(const oshape* A, const oshape* B, const tr3* a2w, const tr3
       if (accel) { accel = 0; continue; }
       break;
     }
-    for (int i = 3; i > 0; --i) { S.y[i] = S.y[i - 1]; S.p[i] = S.p[i - 1]; S.q[i] = S.q[i - 1]; }
+    /* the simplex holds at most 3 points here (a tetrahedron either
+     * encloses the origin or is reduced), so the shift loses nothing and can
+     * be undone */
+    for (int i = 3; i > 0; --i) {
+      S.y[i] = S.y[i - 1];
+      S.p[i] = S.p[i - 1];
+      S.q[i] = S.q[i - 1];
+      S.lam[i] = S.lam[i - 1];
+    }
     S.y[0] = w;
     S.p[0] = pl;
     S.q[0] = ql;
-    S.n = S.n < 4 ? S.n + 1 : 4;
+    S.lam[0] = 0.0;
+    const int n_prev = S.n;
+    S.n = S.n + 1;
     ++num_iter;
     sv_res r;
     if (S.n == 1) {
@@ -889,18 +917,31 @@ static double gjk_sv(const oshape* A, const oshape* B, const tr3* a2w, const tr3
     } else {
       r = sv_tet(&S);
     }
-    sv_compact(&S, &r);
-    if (r.mask == 15u) { inside = 1; dist = 0.0; break; }
-    double nd = vnorm(r.v);
-    wprev = w;
-    /* no progress: rounding has the last word.  The first support point is
-     * exempt: v0 (the centre difference) is not a point of the simplex. */
-    if (num_iter > 1 && !(nd < dist)) {
-      if (accel) { accel = 0; v = r.v; dist = nd; continue; }
-      v = r.v;
-      dist = nd;
+    if (r.mask == 15u) {
+      sv_compact(&S, &r);
+      inside = 1;
+      dist = 0.0;
       break;
     }
+    double nd = vnorm(r.v);
+    /* No progress: rounding has the last word (a thin triangle's projection
+     * can come out worse than its edge).  Drop w and restore the previous
+     * simplex, whose |v| is the best bound found; an accelerated search
+     * continues unaccelerated, a plain one stops.  The first support point
+     * is exempt: v0 (the centre difference) is not a point of the simplex. */
+    if (num_iter > 1 && !(nd < dist)) {
+      for (int i = 0; i < 3; ++i) {
+        S.y[i] = S.y[i + 1];
+        S.p[i] = S.p[i + 1];
+        S.q[i] = S.q[i + 1];
+        S.lam[i] = S.lam[i + 1];
+      }
+      S.n = n_prev;
+      if (accel) { accel = 0; continue; }
+      break;
+    }
+    sv_compact(&S, &r);
+    wprev = w;
     v = r.v;
     dist = nd;
   }
@@ -920,19 +961,12 @@ static double gjk_sv(const oshape* A, const oshape* B, const tr3* a2w, const tr3
   return dist / sc;
 }
 
-/* computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391).
- * REF_QUIRKS: exactly the reference, incl. v0 = s_A(0) - b2a(s_B(0)) and the
- * break on mu < 1e-4*relError.  Physics: v0 = centre difference, no absolute
- * break, supports of the crust cores, witness points of the last support
- * pair when the simplex is empty; NESTEROV adds momentum on the support
- * direction (Montaut et al., RSS 2022), with the van den Bergen lower bound
- * d.w/|d| valid for any direction d. */
+/* computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391), exactly the reference
+ * (the REF_QUIRKS semantics): full shapes, v0 = s_A(0) - b2a(s_B(0)), the
+ * break on mu < 1e-4*relError, the stale |v| kept when Johnson's backup
+ * procedure is taken.  The physics semantics use gjk_sv. */
 static double gjk_jh(const oshape* A, const oshape* B, const tr3* a2w, const tr3* b2w,
-                     uint32_t flags, double rel_tol, v3* pa, v3* pb, int* nb_iter,
-                     int* deep) {
-  const int quirks = (flags & GRAINS_REF_QUIRKS) != 0;
-  const int core = !quirks;
-  const int nesterov = !quirks && (flags & GRAINS_NESTEROV);
+                     double rel_tol, v3* pa, v3* pb, int* nb_iter) {
   unsigned bits = 0, last = 0, last_bit = 0, all_bits = 0;
   jh_state S;
   memset(&S, 0, sizeof(S));
@@ -941,27 +975,9 @@ static double gjk_jh(const oshape* A, const oshape* B, const tr3* a2w, const tr3
   const double rel = rel_tol > 0 ? rel_tol : LOWEPS;
   const double abs_err = 1.e-4 * rel;
   tr3 b2a = trelative(a2w, b2w);
-  /* Physics semantics work in the pair's unit frame: every support point and
-   * b2a's offset is multiplied by the power of two sc with
-   * (R_A + R_B) * sc in [0.5, 1).  The sub-algorithm's absolute thresholds
-   * (det <= EPS) then see determinants of order one whatever the unit of
-   * length; a power of two scales without rounding. */
-  double sc = 1.0;
-  if (!quirks) {
-    int e;
-    frexp(A->rc_exact + B->rc_exact, &e);
-    sc = ldexp(1.0, -e);
-    b2a.o = vscale(sc, b2a.o);
-  }
-  v3 v;
-  if (quirks)
-    v = vsub(support(A, V(0, 0, 0), 0), tapply(&b2a, support(B, V(0, 0, 0), 0)));
-  else
-    v = vneg(b2a.o);
+  v3 v = vsub(support(A, V(0, 0, 0), 0), tapply(&b2a, support(B, V(0, 0, 0), 0)));
   v3 w = V(0, 0, 0);
   double dist = vnorm(v);
-  v3 dir = v, wprev = v; /* Nesterov direction memory */
-  int accel = nesterov;
   while (bits < 15 && dist > HIGHEPS && num_iter < 1000) {
     last = 0;
     last_bit = 1;
@@ -969,67 +985,21 @@ static double gjk_jh(const oshape* A, const oshape* B, const tr3* a2w, const tr3
       ++last;
       last_bit <<= 1;
     }
-    v3 d = v;
-    if (accel) {
-      double delta = (double)(num_iter + 1) / (double)(num_iter + 3);
-      v3 yk = vadd(vscale(delta, v), vscale(1.0 - delta, wprev));
-      double ny = vnorm(yk), nd = vnorm(dir);
-      if (num_iter == 0 || ny <= HIGHEPS || nd <= HIGHEPS)
-        d = v;
-      else
-        d = vadd(vscale(delta / nd, dir), vscale((1.0 - delta) / ny, yk));
-      dir = d;
-    }
-    S.p[last] = vscale(sc, support(A, vneg(d), core));
-    S.q[last] = vscale(sc, support(B, vm(d, &b2a.R), core));
+    S.p[last] = support(A, vneg(v), 0);
+    S.q[last] = support(B, vm(v, &b2a.R), 0);
     w = vsub(S.p[last], tapply(&b2a, S.q[last]));
-    /* optimality gap: lower bound d.w/|d| (= v.w/dist without momentum) */
-    double lb = accel ? vdot(d, w) / vnorm(d) : vdot(v, w) / dist;
+    double lb = vdot(v, w) / dist;
     if (mu < lb) mu = lb;
-    if (quirks) {
-      if (dist - mu <= dist * rel || mu < abs_err) break;
-    } else if (dist - mu <= dist * rel)
-      break;
-    if (jh_degenerate(all_bits, S.y, w)) {
-      if (accel) { /* momentum stalled: continue as plain GJK */
-        accel = 0;
-        continue;
-      }
-      break;
-    }
+    if (dist - mu <= dist * rel || mu < abs_err) break;
+    if (jh_degenerate(all_bits, S.y, w)) break;
     S.y[last] = w;
     all_bits = bits | last_bit;
     ++num_iter;
-    double dist_before = dist;
-    if (!jh_closest(&S, &bits, last, last_bit, all_bits, &v)) {
-      /* backup sub-simplex chosen: the reference keeps the stale |v| and
-       * stops; the physics semantics use the returned simplex's |v|, and an
-       * accelerated search continues from it as plain GJK */
-      if (!quirks) dist = vnorm(v);
-      if (accel) {
-        accel = 0;
-        wprev = w;
-        continue;
-      }
-      break;
-    }
+    if (!jh_closest(&S, &bits, last, last_bit, all_bits, &v)) break;
     dist = vnorm(v);
-    wprev = w;
-    if (accel && dist > dist_before * (1.0 - 1e-12)) accel = 0;
   }
-  if (!quirks && bits == 0) {
-    /* no simplex: witnesses from the last support pair (or the centres) */
-    *pa = num_iter || last_bit ? S.p[last] : V(0, 0, 0);
-    *pb = num_iter || last_bit ? S.q[last] : V(0, 0, 0);
-  } else
-    jh_points(bits, S.det, S.p, S.q, pa, pb);
-  if (deep) *deep = (!quirks && (bits == 15 || dist <= HIGHEPS));
+  jh_points(bits, S.det, S.p, S.q, pa, pb);
   *nb_iter = num_iter;
-  if (!quirks) { /* back to the caller's unit of length */
-    *pa = vscale(1.0 / sc, *pa);
-    *pb = vscale(1.0 / sc, *pb);
-    dist /= sc;
-  }
   return dist;
 }
 
@@ -1077,7 +1047,8 @@ typedef struct {
 
 /* closestPointsRigidBodies (CollisionDetection.cpp:181-284).  With physics
  * semantics GJK runs on the crust cores (the commented-out intent of lines
- * 231-236) with the fixed termination of gjk_jh. */
+ * 231-236) with the signed-volume GJK (gjk_sv); REF_QUIRKS runs the
+ * reference's Johnson GJK on the full shapes. */
 static cinfo closest_points(const oshape* A, const oshape* B, const tr3* a2w, const tr3* b2w,
                             uint32_t flags, double rel_tol, np_counters* cnt) {
   if (A->s.kind == GRAINS_SPHERE && B->s.kind == GRAINS_SPHERE)
@@ -1091,7 +1062,7 @@ static cinfo closest_points(const oshape* A, const oshape* B, const tr3* a2w, co
       v3 ptA, ptB;
       int it = 0, deep = 0;
       double distance = (flags & GRAINS_REF_QUIRKS)
-                            ? gjk_jh(A, B, a2w, b2w, flags, rel_tol, &ptA, &ptB, &it, &deep)
+                            ? gjk_jh(A, B, a2w, b2w, rel_tol, &ptA, &ptB, &it)
                             : gjk_sv(A, B, a2w, b2w, flags, rel_tol, &ptA, &ptB, &it, &deep);
       if (cnt) {
         cnt->gjk_calls++;
@@ -1804,7 +1775,7 @@ double go_gjk(const grains_shape_t* a, const grains_shape_t* b, const double* ve
   tr3 TA = tr_from12(ta), TB = tr_from12(tb);
   v3 p, q;
   int it = 0;
-  double d = (flags & GRAINS_REF_QUIRKS) ? gjk_jh(&A, &B, &TA, &TB, flags, rel_tol, &p, &q, &it, NULL)
+  double d = (flags & GRAINS_REF_QUIRKS) ? gjk_jh(&A, &B, &TA, &TB, rel_tol, &p, &q, &it)
                                          : gjk_sv(&A, &B, &TA, &TB, flags, rel_tol, &p, &q, &it, NULL);
   p = tapply(&TA, p);
   q = tapply(&TB, q);
