@@ -35,9 +35,12 @@ PEAK_FP64_TFLOPS = 78.6
 # Algorithmic operations of the narrow-phase kernel, counted from the source
 # (gr_geom.hpp; transcendental = 1 op, FMA = 2 ops): one GJK iteration = two
 # log-domain superquadric supports (2 x 42), the rotation of the direction
-# and of B's support (33), gap test and bookkeeping (12), Johnson's
-# sub-algorithm (~70 on average over simplex sizes) and |v| (6).
-OPS_PER_GJK_ITER = 205
+# into B's frame and of B's support into A's, with the unit-frame scaling
+# (42), lower bound, separation and gap tests (10), the signed-volume
+# sub-algorithm (segment 20, triangle 64 + its edge fall-backs, tetrahedron
+# 62 + its face fall-backs: ~80 on average over the simplex sizes met) and
+# |v| (6).
+OPS_PER_GJK_ITER = 222
 # per pair reaching GJK: two rotation matrices (2 x 30), the relative
 # transform (63), OBB test (110), support set-up (2 x 16), witness points and
 # contact vector (45), Hooke force when in contact (counted always, 80)
@@ -59,10 +62,10 @@ def log(*a):
 GJK_TOL = {A.FP64: 1e-10, A.FP32_MIXED: 1e-6}
 
 
-def run_engine(sc, precision, warmup, steps, profile=True, flags=0):
+def run_engine(sc, precision, warmup, steps, profile=True, flags=0, gjk_tol=None):
     from grains3dgpu_amd.engine import Engine
-    e = S.load(Engine(dt=sc.dt, precision=precision, max_neighbors=64, gjk_rel_tol=GJK_TOL[precision],
-                      flags=flags), sc)
+    tol = GJK_TOL[precision] if gjk_tol is None else gjk_tol
+    e = S.load(Engine(dt=sc.dt, precision=precision, max_neighbors=64, gjk_rel_tol=tol, flags=flags), sc)
     if warmup:
         e.step(warmup)
     e.set_profiling(profile)
@@ -127,6 +130,20 @@ def gjk_microbench(precision):
                 finite=bool(np.isfinite(d).all()))
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain copy kernel of known size: each dispatch reads and writes
+    nbytes.  FETCH_SIZE / WRITE_SIZE per dispatch of this run, divided by
+    nbytes / 1024, give the factors that turn the counters of the step's
+    kernels into HBM bytes."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_dispatch": nbytes, "dispatches": reps}), flush=True)
+
+
 def cpu_baseline(seconds, threads=1):
     from oracle import pyoracle
     sc = S.config1(n=8000, seed=7)
@@ -154,7 +171,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (reference for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the batched multi-simulation form is the way to fill one device", file=sys.stderr)
@@ -194,10 +216,15 @@ def main():
     ops = OPS_PER_GJK_ITER * st["gjk_iterations"] / K + OPS_PER_GJK_PAIR * st["gjk_calls"] / K + \
         OPS_PER_ENTRY * st["candidate_pairs"] / K
     achieved = ops / (nar_ms * 1e-3) / 1e12
+    # HBM bytes per launch of k_narrow<float>, from the FETCH_SIZE and
+    # WRITE_SIZE counter runs summarised in profiles/ (a line
+    # "hbm_bytes_per_launch = N"); null until measured
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "narrow32_c1_1M_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "narrow32_c1_1M_traffic.txt")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        for line in open(tf):
+            if line.strip().startswith("hbm_bytes_per_launch"):
+                traffic = float(line.split("=")[1])
     roofline = dict(bound="valu", kernel="k_narrow<float>", achieved=achieved, peak=PEAK_FP32_TFLOPS,
                     unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic,
                     avg_launch_ms=nar_ms, ops_per_launch=ops,
@@ -226,12 +253,20 @@ def main():
         log(f"agitated bed, {na} particles: FP32-mixed and FP64, {W + K} steps each")
         a32 = run_engine(sa, A.FP32_MIXED, W, K, profile=False)
         a64 = run_engine(sa, A.FP64, W, K, profile=False)
+        # the same FP64 path stopped at the FP32 path's GJK gap: the part of
+        # the divergence that the gap alone causes in this chaotic flow
+        a64g = run_engine(sa, A.FP64, W, K, profile=False, gjk_tol=GJK_TOL[A.FP32_MIXED])
         d = np.abs(a32["pos"] - a64["pos"]).max()
+        dg = np.abs(a64g["pos"] - a64["pos"]).max()
         disp = np.abs(a64["pos"] - sa.pos).max()
+        vmax = np.abs(a64["vel"]).max()
         agit = dict(n=na, steps=W + K, contacts_fp64=a64["stats"]["contacts"],
                     contacts_fp32=a32["stats"]["contacts"], pos_rel=float(d / np.abs(a64["pos"]).max()),
                     pos_rel_displacement=float(d / max(disp, 1e-300)), max_displacement_m=float(disp),
-                    vel_rel=float(np.abs(a32["vel"] - a64["vel"]).max() / np.abs(a64["vel"]).max()),
+                    vel_rel=float(np.abs(a32["vel"] - a64["vel"]).max() / vmax),
+                    fp64_at_fp32_gap=dict(pos_rel=float(dg / np.abs(a64["pos"]).max()),
+                                          pos_rel_displacement=float(dg / max(disp, 1e-300)),
+                                          vel_rel=float(np.abs(a64g["vel"] - a64["vel"]).max() / vmax)),
                     fp32_particle_steps_per_s=na * K / a32["wall_s"],
                     fp64_particle_steps_per_s=na * K / a64["wall_s"])
         log(f"  error {agit}")

@@ -398,6 +398,7 @@ struct GjkOut {
   bool deep;      // physics semantics: crust cores overlap
   bool backup;    // Johnson: the backup procedure was taken; signed volumes:
                   // the search ended on rounding or the cap, not the gap test
+  bool separated; // signed volumes: the lower bound passed `sep` (no contact)
 };
 
 // computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391), exactly the reference
@@ -480,6 +481,7 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   out.iters = num_iter;
   out.deep = false;
   out.backup = backup;
+  out.separated = false;
   return out;
 }
 
@@ -651,9 +653,14 @@ constexpr int kNesterovIters = 8;
 // support directions (Montaut et al. 2022).  `backup` reports a search that
 // ended on rounding (a repeated support point or no decrease of |v|) or on
 // the iteration cap instead of the gap test.
+// Separation early-out: once the lower bound mu exceeds `sep` (the crust
+// sum) the cores are farther apart than the crusts can bridge and the search
+// stops with `separated` (INFINITY for a plain distance query).  Contact
+// decisions equal those of the full search, which reports no contact exactly
+// when its distance, an upper bound, is above sep -- implied by mu > sep.
 template <class T>
 GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
-                      const GjkArgs& args, T rsum) {
+                      const GjkArgs& args, T rsum, T sep) {
   const bool nesterov = (args.flags & GRAINS_NESTEROV) != 0;
   const T rel = args.rel > 0 ? T(args.rel) : Eps<T>::LOW;
   Tr<T> b2a = relative(a2w, b2w);
@@ -661,6 +668,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   frexp(rsum, &ex);
   const T sc = ldexp(T(1), -ex);
   b2a.o = sc * b2a.o;
+  const T sep_s = sc * sep;
   const V3<T> zero = {T(0), T(0), T(0)};
   SvSimplex<T> S;
 #pragma unroll
@@ -672,7 +680,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   V3<T> v = -b2a.o, w = zero, pl = zero, ql = zero;
   T dist = norm(v), mu = T(0);
   V3<T> dir = v, wprev = v;
-  bool accel = nesterov, inside = false, ran = false, stalled = false;
+  bool accel = nesterov, inside = false, ran = false, stalled = false, separated = false;
   int num_iter = 0;
   while (dist > Eps<T>::HIGH && num_iter < 1000) {
     V3<T> d = v;
@@ -691,6 +699,10 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
     ran = true;
     const T lb = accel ? dot(d, w) / norm(d) : dot(v, w) / dist;
     if (mu < lb) mu = lb;
+    if (mu > sep_s) {
+      separated = true;
+      break;
+    }
     if (dist - mu <= dist * rel) break;
     bool degen = false;
 #pragma unroll
@@ -787,6 +799,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   out.iters = num_iter;
   out.deep = inside || dist <= Eps<T>::HIGH;
   out.backup = stalled;
+  out.separated = separated;
   return out;
 }
 
@@ -847,15 +860,16 @@ GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<
     if (obb_intersect(A.bbox, B.bbox, a2w, b2w)) {
       SupCtx<T> ca = sup_ctx(A, !quirks, verts_full, verts_core);
       SupCtx<T> cb = sup_ctx(B, !quirks, verts_full, verts_core);
+      const T ctSum = A.crust + B.crust;
       GjkOut<T> g;
       if constexpr (SV)
-        g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc);
+        g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum);
       else
         g = gjk_jh(ca, cb, a2w, b2w, args);
       st.gjk = 1;
       st.iters = g.iters;
       st.backup = g.backup;
-      T ctSum = A.crust + B.crust;
+      if (g.separated) return no_contact<T>();
       T distance = g.dist - ctSum;
       if (g.deep) distance = -ctSum;
       if (distance > T(0)) return no_contact<T>();

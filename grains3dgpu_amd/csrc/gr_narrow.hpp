@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(kBlock) k_gjk_batch(const ShapeNP<T>* __restri
   SupCtx<T> cb = sup_ctx(B, false, verts, verts);
   GjkOut<T> g;
   if constexpr (SV)
-    g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc);
+    g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc, T(INFINITY));
   else
     g = gjk_jh(ca, cb, a2w, b2w, args);
   dist[k] = (double)g.dist;

@@ -845,9 +845,17 @@ static void sv_compact(sv_simplex* S, const sv_res* r) {
 
 /* GJK with the signed-volume sub-algorithm, physics semantics: crust cores,
  * the pair's unit frame (power-of-two scale), v0 = centre difference, the
- * van den Bergen gap test, NESTEROV momentum on the support direction. */
+ * van den Bergen gap test, NESTEROV momentum on the support direction.
+ * Separation early-out: once the lower bound mu exceeds `sep` (the crust
+ * sum, in the caller's unit) the cores are farther apart than the crusts can
+ * bridge, the pair has no contact whatever the exact distance, and the
+ * search stops with *separated = 1 (pass INFINITY for a plain distance
+ * query).  Every contact decision equals that of the full search: the full
+ * search reports no contact exactly when its distance, an upper bound, is
+ * above sep, and mu > sep implies it. */
 static double gjk_sv(const oshape* A, const oshape* B, const tr3* a2w, const tr3* b2w,
-                     uint32_t flags, double rel_tol, v3* pa, v3* pb, int* nb_iter, int* deep) {
+                     uint32_t flags, double rel_tol, double sep, v3* pa, v3* pb, int* nb_iter,
+                     int* deep, int* separated) {
   const int nesterov = (flags & GRAINS_NESTEROV) != 0;
   const double rel = rel_tol > 0 ? rel_tol : LOWEPS;
   tr3 b2a = trelative(a2w, b2w);
@@ -855,12 +863,14 @@ static double gjk_sv(const oshape* A, const oshape* B, const tr3* a2w, const tr3
   frexp(A->rc_exact + B->rc_exact, &e);
   const double sc = ldexp(1.0, -e);
   b2a.o = vscale(sc, b2a.o);
+  const double sep_s = sc * sep;
   sv_simplex S;
   memset(&S, 0, sizeof(S));
   v3 v = vneg(b2a.o), w = V(0, 0, 0), pl = V(0, 0, 0), ql = V(0, 0, 0);
   double dist = vnorm(v), mu = 0.0;
   v3 dir = v, wprev = v;
   int accel = nesterov, num_iter = 0, inside = 0, ran = 0;
+  *separated = 0;
   while (dist > HIGHEPS && num_iter < 1000) {
     v3 d = v;
     /* momentum on the first SV_NESTEROV_ITERS support points only: near the
@@ -880,6 +890,10 @@ static double gjk_sv(const oshape* A, const oshape* B, const tr3* a2w, const tr3
     ran = 1;
     double lb = accel ? vdot(d, w) / vnorm(d) : vdot(v, w) / dist;
     if (mu < lb) mu = lb;
+    if (mu > sep_s) {
+      *separated = 1;
+      break;
+    }
     if (dist - mu <= dist * rel) break;
     int degen = 0;
     for (int i = 0; i < S.n; ++i)
@@ -1060,15 +1074,17 @@ static cinfo closest_points(const oshape* A, const oshape* B, const tr3* a2w, co
   if (overlap < 0.0) {
     if (obb_intersect(A->bbox, B->bbox, a2w, b2w)) {
       v3 ptA, ptB;
-      int it = 0, deep = 0;
+      int it = 0, deep = 0, separated = 0;
+      double ctSum = A->s.crust + B->s.crust;
       double distance = (flags & GRAINS_REF_QUIRKS)
                             ? gjk_jh(A, B, a2w, b2w, rel_tol, &ptA, &ptB, &it)
-                            : gjk_sv(A, B, a2w, b2w, flags, rel_tol, &ptA, &ptB, &it, &deep);
+                            : gjk_sv(A, B, a2w, b2w, flags, rel_tol, ctSum, &ptA, &ptB, &it, &deep,
+                                     &separated);
       if (cnt) {
         cnt->gjk_calls++;
         cnt->gjk_iters += (uint64_t)it;
       }
-      double ctSum = A->s.crust + B->s.crust;
+      if (separated) return NO_CONTACT;
       distance -= ctSum;
       if (deep) distance = -ctSum; /* cores overlap: deepest depth the crusts allow */
       if (distance > 0.0) return NO_CONTACT;
@@ -1774,9 +1790,10 @@ double go_gjk(const grains_shape_t* a, const grains_shape_t* b, const double* ve
   if (B.core_verts) memcpy(B.core_verts, B.verts, sizeof(double) * 3 * B.nv);
   tr3 TA = tr_from12(ta), TB = tr_from12(tb);
   v3 p, q;
-  int it = 0;
+  int it = 0, sep = 0;
   double d = (flags & GRAINS_REF_QUIRKS) ? gjk_jh(&A, &B, &TA, &TB, rel_tol, &p, &q, &it)
-                                         : gjk_sv(&A, &B, &TA, &TB, flags, rel_tol, &p, &q, &it, NULL);
+                                         : gjk_sv(&A, &B, &TA, &TB, flags, rel_tol, INFINITY, &p, &q, &it,
+                                                  NULL, &sep);
   p = tapply(&TA, p);
   q = tapply(&TB, q);
   if (pa) { pa[0] = p.x; pa[1] = p.y; pa[2] = p.z; }
