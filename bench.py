@@ -47,6 +47,14 @@ OPS_PER_GJK_ITER = 222
 OPS_PER_GJK_PAIR = 390
 # per candidate entry: loads, periodic image, bounding-sphere test (20)
 OPS_PER_ENTRY = 20
+# Compulsory HBM bytes of one narrow-phase launch (k_narrow): each particle's
+# pose (pos 24 + quat 32, FP64) and shape and sim ids (8) once, each shape's
+# narrow-phase record (ShapeNP<float>, 96) once, each candidate entry's two
+# ids (8) and flag (4), each contact's force and torque (FP32, 24)
+B_NP_PARTICLE = 64
+B_NP_SHAPE = 96
+B_NP_ENTRY = 12
+B_NP_CONTACT = 24
 # Compulsory HBM bytes per particle-step of the FP32-mixed step at 3 radix
 # passes (state read 84 + write 80, key+index 8, sort 16 x 3, sorted key 4)
 B_PS_FP32 = 224
@@ -213,9 +221,11 @@ def main():
 
     st = r32["stats"]
     nar_ms = r32["phases"]["narrow"] / K
-    ops = OPS_PER_GJK_ITER * st["gjk_iterations"] / K + OPS_PER_GJK_PAIR * st["gjk_calls"] / K + \
-        OPS_PER_ENTRY * st["candidate_pairs"] / K
-    achieved = ops / (nar_ms * 1e-3) / 1e12
+    E, C = st["candidate_pairs"] / K, st["contacts"] / K
+    n_shapes = len(np.unique(sc.particle_shape))
+    # compulsory HBM bytes of one k_narrow launch
+    nbytes = sc.n * B_NP_PARTICLE + n_shapes * B_NP_SHAPE + E * B_NP_ENTRY + C * B_NP_CONTACT
+    achieved_gbs = nbytes / (nar_ms * 1e-3) / 1e9
     # HBM bytes per launch of k_narrow<float>, from the FETCH_SIZE and
     # WRITE_SIZE counter runs summarised in profiles/ (a line
     # "hbm_bytes_per_launch = N"); null until measured
@@ -225,11 +235,18 @@ def main():
         for line in open(tf):
             if line.strip().startswith("hbm_bytes_per_launch"):
                 traffic = float(line.split("=")[1])
-    roofline = dict(bound="valu", kernel="k_narrow<float>", achieved=achieved, peak=PEAK_FP32_TFLOPS,
-                    unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic,
-                    avg_launch_ms=nar_ms, ops_per_launch=ops,
-                    note="narrow phase is VALU/transcendental-bound (no MFMA: no dense contraction); "
-                         "peak = FP32 vector")
+    roofline = dict(bound="hbm", kernel="k_narrow<float,true>", achieved=achieved_gbs, peak=PEAK_HBM_GBS,
+                    unit="GB/s", frac=achieved_gbs / PEAK_HBM_GBS, traffic=traffic,
+                    bytes_per_launch=nbytes, avg_launch_ms=nar_ms,
+                    note="compulsory bytes: 64 B pose+ids per particle, 96 B per shape record, 12 B per "
+                         "candidate entry, 24 B per contact; the launch is gather-latency bound "
+                         "(1.1 GJK iterations per query after the separation early-out)")
+    ops = OPS_PER_GJK_ITER * st["gjk_iterations"] / K + OPS_PER_GJK_PAIR * st["gjk_calls"] / K + \
+        OPS_PER_ENTRY * st["candidate_pairs"] / K
+    achieved = ops / (nar_ms * 1e-3) / 1e12
+    roofline_valu = dict(bound="valu", kernel="k_narrow<float,true>", achieved=achieved, peak=PEAK_FP32_TFLOPS,
+                         unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS, ops_per_launch=ops,
+                         note="no MFMA: no dense contraction on the path; peak = FP32 vector")
     step_ms = r32["wall_s"] * 1e3 / K
     roofline_step = dict(bound="hbm", bytes_per_particle_step=B_PS_FP32,
                          achieved=B_PS_FP32 * ps32 / 1e9, peak=PEAK_HBM_GBS, unit="GB/s",
@@ -310,6 +327,7 @@ def main():
         "phase_ms_per_step": {k: v / K for k, v in r32["phases"].items()},
         "gjk_microbench": {"fp64": mb64, "fp32": mb32},
         "roofline": roofline,
+        "roofline_valu": roofline_valu,
         "roofline_step_hbm": roofline_step,
         "cpu_baseline": cpu,
     }
