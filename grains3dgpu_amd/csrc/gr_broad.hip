@@ -7,12 +7,13 @@
 //                 with wave ballots (replaces thrust::sort_by_key,
 //                 ComponentManagerGPU.cpp:282-285)
 //   k_cells       cell start/end by adjacent difference
-//                 (ComponentManagerGPU_Kernels.cpp:28-75) and the inverse
-//                 permutation
+//                 (ComponentManagerGPU_Kernels.cpp:28-75), the inverse
+//                 permutation and the pose records in cell order
 //   k_candidates  per particle, the 27 cells in the reference order
 //                 (LinkedCell.cpp:41-70) and their residents ascending
 //                 (ComponentManagerCPU.cpp:364-421), bounding-sphere prefilter
-//   scan/expand   CSR offsets and the flat (i, j) entry list
+//   scan/expand   CSR offsets and the flat (i, j) entry list, in sorted
+//                 positions
 #include "gr_internal.hpp"
 
 namespace gr {
@@ -239,44 +240,64 @@ void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx
   *idx_sorted = const_cast<uint32_t*>(iin);
 }
 
-// ---- cell ranges --------------------------------------------------------------------
-__global__ void k_cells(uint64_t n, const uint32_t* __restrict__ ks, const uint32_t* __restrict__ is,
-                        uint32_t* __restrict__ cs, uint32_t* __restrict__ ce, uint32_t* __restrict__ rank) {
-  uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// ---- cell ranges and sorted records -----------------------------------------------
+// One thread per sorted position s: cell start/end by adjacent difference,
+// the inverse permutation, and the particle's records in cell order -- the
+// only gather of the state per step; the candidate and narrow phases read
+// bp / pk of neighbouring cells, which are neighbouring lines.
+__global__ void k_cells(DevScene d, const uint32_t* __restrict__ ks, const uint32_t* __restrict__ is) {
+  const uint64_t n = d.n;
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  uint32_t k = ks[s];
-  if (s == 0 || ks[s - 1] != k) cs[k] = (uint32_t)s;
-  if (s == n - 1 || ks[s + 1] != k) ce[k] = (uint32_t)s + 1;
-  rank[is[s]] = (uint32_t)s;
+  const uint32_t k = ks[s];
+  if (s == 0 || ks[s - 1] != k) d.cell_start[k] = (uint32_t)s;
+  if (s == n - 1 || ks[s + 1] != k) d.cell_end[k] = (uint32_t)s + 1;
+  const uint32_t i = is[s];
+  d.rank[i] = (uint32_t)s;
+  const uint32_t shp = d.shape[i];
+  PoseRec r;
+  r.x = d.pos[i];
+  r.y = d.pos[n + i];
+  r.z = d.pos[2 * n + i];
+  r.qx = d.quat[i];
+  r.qy = d.quat[n + i];
+  r.qz = d.quat[2 * n + i];
+  r.qw = d.quat[3 * n + i];
+  r.shape = shp;
+  r.sim = d.sim[i];
+  d.pk[s] = r;
+  d.bp[s] = make_double4(r.x, r.y, r.z, d.rc_pf[shp]);
 }
 
 void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
   // empty cells: start = end = 0
   launch_fill_u32(d.cell_start, d.ncells, 0u, s);
   launch_fill_u32(d.cell_end, d.ncells, 0u, s);
-  hipLaunchKernelGGL(k_cells, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d.n,
-                     keys_s, idx_s, d.cell_start, d.cell_end, d.rank);
+  hipLaunchKernelGGL(k_cells, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d, keys_s,
+                     idx_s);
 }
 
 // ---- candidates -------------------------------------------------------------------
 // One thread per sorted position s (spatially coherent waves).  Writes the
-// candidates of particle idx_s[s] into cand[s*K ...] in the reference visit
-// order and their count into cand_count[s].
-__global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_t* __restrict__ ks,
-                                                      const uint32_t* __restrict__ is) {
+// sorted positions of its candidates, in the reference visit order, into
+// cand[k * N + s] (k-major, so the lanes of a wave write neighbouring words)
+// and their count into cand_count[s].  Residents of a cell ascend in sorted
+// position exactly as in particle index (the sort is stable on an identity
+// payload).
+__global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_t* __restrict__ ks) {
   const uint64_t n = d.n;
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t cnt = 0, over = 0;
   if (s < n) {
-    const uint32_t i = is[s];
-    const SimGrid g = d.grids[d.sim[i]];
-    const V3<double> xi = load3(d.pos, n, i);
-    const double ri = d.rc_pf[d.shape[i]];
+    const double4 bi = d.bp[s];
+    const SimGrid g = d.grids[d.pk[s].sim];
+    const V3<double> xi = {bi.x, bi.y, bi.z};
+    const double ri = bi.w;
     const uint32_t local = ks[s] - g.base;
     const int cx = (int)(local % g.n3[0]);
     const int cy = (int)((local / g.n3[0]) % g.n3[1]);
     const int cz = (int)(local / (g.n3[0] * g.n3[1]));
-    uint32_t* out = d.cand + s * d.K;
+    const uint32_t K = d.K;
     for (int dz = -1; dz <= 1; ++dz) {
       int z = cz + dz;
       if (g.periodic & 4) z = (z + (int)g.n3[2]) % (int)g.n3[2];
@@ -292,14 +313,14 @@ __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_
           const uint32_t cell = g.base + ((uint32_t)z * g.n3[1] + (uint32_t)y) * g.n3[0] + (uint32_t)x;
           const uint32_t t1 = d.cell_end[cell];
           for (uint32_t t = d.cell_start[cell]; t < t1; ++t) {
-            const uint32_t j = is[t];
-            if (j == i) continue;
-            V3<double> dd = load3(d.pos, n, j) - xi;
+            if (t == (uint32_t)s) continue;
+            const double4 bj = d.bp[t];
+            V3<double> dd = V3<double>{bj.x, bj.y, bj.z} - xi;
             if (g.periodic) dd = min_image(g, dd);
-            const double rs = ri + d.rc_pf[d.shape[j]];
+            const double rs = ri + bj.w;
             if (!(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z < rs * rs * (1.0 + kPrefilterSlack))) continue;
-            if (cnt < d.K)
-              out[cnt] = j;
+            if (cnt < K)
+              d.cand[(uint64_t)cnt * n + s] = t;
             else
               over = 1;
             ++cnt;
@@ -307,40 +328,39 @@ __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_
         }
       }
     }
-    d.cand_count[s] = cnt < d.K ? cnt : d.K;
+    d.cand_count[s] = cnt < K ? cnt : K;
   }
   count_add(&d.counters[kCntOverflow], over);
   uint32_t m = wave_max(cnt);
   if (__lane_id() == 0 && m) atomicMax((unsigned long long*)&d.counters[kCntMaxCand], (unsigned long long)m);
 }
 
-void launch_candidates(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
+void launch_candidates(const DevScene& d, const uint32_t* keys_s, hipStream_t s) {
   hipLaunchKernelGGL(k_candidates, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d,
-                     keys_s, idx_s);
+                     keys_s);
 }
 
-// flat entry list: entry e = entry_off[s] + k holds (idx_s[s], cand[s*K + k])
-__global__ void k_expand(uint64_t n, uint32_t K, const uint32_t* __restrict__ is,
-                         const uint32_t* __restrict__ cand, const uint32_t* __restrict__ cnt,
-                         const uint32_t* __restrict__ off, uint32_t* __restrict__ pi,
-                         uint32_t* __restrict__ pj, unsigned long long* __restrict__ counters) {
+// flat entry list in sorted positions: entry e = entry_off[s] + k holds
+// (s, cand[k * N + s])
+__global__ void k_expand(uint64_t n, const uint32_t* __restrict__ cand, const uint32_t* __restrict__ cnt,
+                         const uint32_t* __restrict__ off, uint32_t* __restrict__ pi, uint32_t* __restrict__ pj,
+                         unsigned long long* __restrict__ counters) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t c = 0;
   if (s < n) {
     c = cnt[s];
-    const uint32_t i = is[s];
     const uint32_t e0 = off[s];
     for (uint32_t k = 0; k < c; ++k) {
-      pi[e0 + k] = i;
-      pj[e0 + k] = cand[s * K + k];
+      pi[e0 + k] = (uint32_t)s;
+      pj[e0 + k] = cand[(uint64_t)k * n + s];
     }
   }
   count_add(&counters[kCntCandidates], c);
 }
 
-void launch_expand(const DevScene& d, const uint32_t* idx_s, hipStream_t s) {
-  hipLaunchKernelGGL(k_expand, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d.n, d.K,
-                     idx_s, d.cand, d.cand_count, d.entry_off, d.pair_i, d.pair_j, d.counters);
+void launch_expand(const DevScene& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_expand, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d.n, d.cand,
+                     d.cand_count, d.entry_off, d.pair_i, d.pair_j, d.counters);
 }
 
 int cu_count() {

@@ -644,6 +644,7 @@ static int finalize(grains_ctx* ctx) {
       (rc = dalloc(ctx, &d.key, n * 4, L)) || (rc = dalloc(ctx, &d.keyA, n * 4, L)) || (rc = dalloc(ctx, &d.idxA, n * 4, L)) ||
       (rc = dalloc(ctx, &d.keyB, n * 4, L)) || (rc = dalloc(ctx, &d.idxB, n * 4, L)) || (rc = dalloc(ctx, &d.rank, n * 4, L)) ||
       (rc = dalloc(ctx, &d.cell_start, (size_t)d.ncells * 4, L)) || (rc = dalloc(ctx, &d.cell_end, (size_t)d.ncells * 4, L)) ||
+      (rc = dalloc(ctx, &d.bp, n * sizeof(double4), L)) || (rc = dalloc(ctx, &d.pk, n * sizeof(PoseRec), L)) ||
       (rc = dalloc(ctx, &d.hist, (256 * ntiles + 1) * 4, L)) || (rc = dalloc(ctx, &d.scan_blocks, nblk * 4, L)) ||
       (rc = dalloc(ctx, &d.cand, ecap * 4, L)) || (rc = dalloc(ctx, &d.cand_count, n * 4, L)) ||
       (rc = dalloc(ctx, &d.entry_off, (n + 1) * 4, L)) || (rc = dalloc(ctx, &d.pair_i, ecap * 4, L)) ||
@@ -818,12 +819,13 @@ static void enqueue_step(grains_ctx* ctx) {
   mark(ctx);
   launch_sort(d, ctx->passes, &ctx->keys_s, &ctx->idx_s, s);
   mark(ctx);
+  d.sidx = ctx->idx_s;
   launch_cells(d, ctx->keys_s, ctx->idx_s, s);
   mark(ctx);
-  launch_candidates(d, ctx->keys_s, ctx->idx_s, s);
+  launch_candidates(d, ctx->keys_s, s);
   mark(ctx);
   launch_scan(d.cand_count, d.entry_off, d.n, d.scan_blocks, s);
-  launch_expand(d, ctx->idx_s, s);
+  launch_expand(d, s);
   mark(ctx);
   if (d.precision == GRAINS_FP64) {
     launch_narrow64(d, s);
@@ -954,9 +956,10 @@ extern "C" int grains_get_pairs(grains_ctx_t* ctx, int which, uint32_t* ij, uint
   if (!ctx->layout_ok || !ctx->idx_s) return fail(ctx, GRAINS_ESTATE, "no step run yet");
   GR_HIP(hipStreamSynchronize(ctx->stream));
   const uint64_t n = ctx->n;
-  std::vector<uint32_t> off(n + 1), rank(n);
+  std::vector<uint32_t> off(n + 1), rank(n), sidx(n);
   GR_HIP(hipMemcpy(off.data(), ctx->d.entry_off, (n + 1) * 4, hipMemcpyDeviceToHost));
   GR_HIP(hipMemcpy(rank.data(), ctx->d.rank, n * 4, hipMemcpyDeviceToHost));
+  GR_HIP(hipMemcpy(sidx.data(), ctx->idx_s, n * 4, hipMemcpyDeviceToHost));
   const uint64_t E = off[n];
   std::vector<uint32_t> pj(E), fl(E);
   if (E) {
@@ -968,9 +971,9 @@ extern "C" int grains_get_pairs(grains_ctx_t* ctx, int which, uint32_t* ij, uint
     const uint32_t s = rank[i];
     for (uint32_t e = off[s]; e < off[s + 1]; ++e) {
       if (which == GRAINS_PAIRS_CONTACT && !(fl[e] & kFlagContact)) continue;
-      if (ij && k < cap) {
+      if (ij && k < cap) {  // entries hold sorted positions
         ij[2 * k] = (uint32_t)i;
-        ij[2 * k + 1] = pj[e];
+        ij[2 * k + 1] = sidx[pj[e]];
       }
       ++k;
     }

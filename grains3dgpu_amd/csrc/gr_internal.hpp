@@ -7,8 +7,13 @@
 //   ids      shape[N], sim[N]                                          u32
 //   broad    key[N] (by particle), key/idx ping-pong [2][N], rank[N],
 //            cell_start[C], cell_end[C]                                u32
-//   pairs    cand[N][K], cand_count[N], entry_off[N+1],
-//            pair_i[E], pair_j[E], pair_flag[E]                        u32
+//   sorted   bp[N] = {x, y, z, prefilter radius}   32 B, cell order
+//            pk[N] = PoseRec {pos, quat, shape, sim}  64 B, cell order
+//            (rebuilt every step from the planes; neighbours of a cell are
+//            contiguous, so the gathers of a wave share cache lines)
+//   pairs    cand[K][N] (k-major: lanes write consecutive words),
+//            cand_count[N], entry_off[N+1]                            u32
+//            pair_i[E], pair_j[E] (sorted positions), pair_flag[E]     u32
 //            pair_res[E] = {F[3], M[3]} in the narrow-phase type
 //   shapes   ShapeNP<double>[S], ShapeNP<float>[S], ShapeDyn[S]
 #pragma once
@@ -72,6 +77,13 @@ struct alignas(16) PairRes {
   T m[3];
 };
 
+// Pose of the particle at one sorted position: one 64-byte line.
+struct alignas(64) PoseRec {
+  double x, y, z;
+  double qx, qy, qz, qw;
+  uint32_t shape, sim;
+};
+
 struct DevScene {
   uint64_t n;
   uint32_t nsims, nobs, nmat;
@@ -94,7 +106,10 @@ struct DevScene {
   const double* obs_tr;         // 12 doubles each
   // broad phase
   uint32_t *key, *keyA, *idxA, *keyB, *idxB, *rank;
+  const uint32_t* sidx;         // particle at each sorted position (this step)
   uint32_t *cell_start, *cell_end;
+  double4* bp;                  // {x, y, z, prefilter radius} by sorted position
+  PoseRec* pk;                  // pose by sorted position
   uint32_t ncells;
   uint32_t *hist;               // 256 * n_tiles (+1)
   uint32_t *scan_blocks;        // block sums of scans
@@ -126,10 +141,11 @@ void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 void launch_hash(const DevScene& d, hipStream_t s);
 // sorts (d.key, identity) into the returned buffers; passes = radix passes
 void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx_sorted, hipStream_t s);
+// cell ranges, inverse permutation and the sorted pose records
 void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s);
-void launch_candidates(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s);
+void launch_candidates(const DevScene& d, const uint32_t* keys_s, hipStream_t s);
 void launch_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* block_sums, hipStream_t s);
-void launch_expand(const DevScene& d, const uint32_t* idx_s, hipStream_t s);
+void launch_expand(const DevScene& d, hipStream_t s);
 int cu_count();
 // gr_narrow64.hip / gr_narrow32.hip
 void launch_narrow64(const DevScene& d, hipStream_t s);

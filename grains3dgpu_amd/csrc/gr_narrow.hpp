@@ -42,24 +42,30 @@ __device__ __forceinline__ void pair_frames(const SimGrid& g, V3<double> xi, Qt<
   }
 }
 
-// Narrow phase + Hooke force of entry (i, j) from i's side
-// (ComponentManagerCPU.cpp:384-419).  Returns the pair flag.
+// Narrow phase + Hooke force of entry (si, sj) -- sorted positions -- from
+// si's side (ComponentManagerCPU.cpp:384-419).  Poses come from the sorted
+// records; velocities, needed only for contacts, from the planes through the
+// particle ids.  Returns the pair flag.
 template <class T, class R, bool SV>
-__device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t i, uint32_t j, PairRes<R>& out,
+__device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, uint32_t sj, PairRes<R>& out,
                                                NpStats& st) {
   const uint64_t n = d.n;
+  const PoseRec pi = d.pk[si];
+  const PoseRec pj = d.pk[sj];
   const ShapeNP<T>* np = np_table<T>(d);
-  const ShapeNP<T> A = np[d.shape[i]];
-  const ShapeNP<T> B = np[d.shape[j]];
-  const SimGrid& g = d.grids[d.sim[i]];
-  const V3<double> xi = load3(d.pos, n, i);
+  const ShapeNP<T> A = np[pi.shape];
+  const ShapeNP<T> B = np[pj.shape];
+  const SimGrid& g = d.grids[pi.sim];
+  const V3<double> xi = {pi.x, pi.y, pi.z};
   Tr<T> a2w, b2w;
-  pair_frames<T>(g, xi, load_quat(d.quat, n, i), load3(d.pos, n, j), load_quat(d.quat, n, j), a2w, b2w);
+  pair_frames<T>(g, xi, Qt<double>{pi.qx, pi.qy, pi.qz, pi.qw}, V3<double>{pj.x, pj.y, pj.z},
+                 Qt<double>{pj.qx, pj.qy, pj.qz, pj.qw}, a2w, b2w);
   const GjkArgs args = {d.flags, d.gjk_rel};
   CInfo<T> ci = closest_points<T, SV>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
   uint32_t flag = (st.gjk ? kFlagGjk : 0u) | (st.deep ? kFlagDeep : 0u);
   if (ci.overlap < T(0)) {
     flag |= kFlagContact;
+    const uint32_t i = d.sidx[si], j = d.sidx[sj];
     const HookeP<T> h = hk_table<T>(d)[g.table + material_hash((uint32_t)A.material, (uint32_t)B.material)];
     const V3<T> vi = cvt<T>(load3(d.vel, n, i)), wi = cvt<T>(load3(d.omega, n, i));
     const V3<T> vj = cvt<T>(load3(d.vel, n, j)), wj = cvt<T>(load3(d.omega, n, j));
