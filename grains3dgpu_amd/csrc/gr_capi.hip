@@ -383,6 +383,8 @@ extern "C" int grains_create(const grains_config_t* cfg, grains_ctx_t** out) {
     return GRAINS_EINVAL;
   }
   if (ctx->cfg.narrow_blocks_per_cu <= 0) ctx->cfg.narrow_blocks_per_cu = 8;
+  if (ctx->cfg.gjk_split_iters == 0) ctx->cfg.gjk_split_iters = 4;
+  if (ctx->cfg.gjk_split_iters >= kGjkMaxIter) ctx->cfg.gjk_split_iters = -1;
   std::memset(&ctx->stats, 0, sizeof(ctx->stats));
   std::memset(&ctx->d, 0, sizeof(ctx->d));
   int ndev = 0;
@@ -650,7 +652,8 @@ static int finalize(grains_ctx* ctx) {
       (rc = dalloc(ctx, &d.entry_off, (n + 1) * 4, L)) || (rc = dalloc(ctx, &d.pair_i, ecap * 4, L)) ||
       (rc = dalloc(ctx, &d.pair_j, ecap * 4, L)) || (rc = dalloc(ctx, &d.pair_flag, ecap * 4, L)) ||
       (rc = dalloc(ctx, &d.pair_res, ecap * res_sz, L)) || (rc = dalloc(ctx, &d.refine_list, ecap * 4, L)) ||
-      (rc = dalloc(ctx, &d.refine_count, 16, L)) || (rc = dalloc(ctx, &d.counters, kNumCounters * 8, L)))
+      (rc = dalloc(ctx, &d.refine_count, 16, L)) || (rc = dalloc(ctx, &d.defer_list, ecap * 4, L)) ||
+      (rc = dalloc(ctx, &d.defer_count, 16, L)) || (rc = dalloc(ctx, &d.counters, kNumCounters * 8, L)))
     return rc;
   GR_HIP(hipMemcpy(p64, np64.data(), S * sizeof(ShapeNP<double>), hipMemcpyHostToDevice));
   GR_HIP(hipMemcpy(p32, np32.data(), S * sizeof(ShapeNP<float>), hipMemcpyHostToDevice));
@@ -691,9 +694,11 @@ static int finalize(grains_ctx* ctx) {
   d.precision = ctx->cfg.precision;
   d.block = (uint32_t)ctx->cfg.block_size;
   d.narrow_blocks = (uint32_t)(cu_count() * ctx->cfg.narrow_blocks_per_cu);
+  d.split = ctx->cfg.gjk_split_iters;
   launch_fill_u32(d.cand_count, n, 0u, ctx->stream);
   launch_fill_u32(d.entry_off, n + 1, 0u, ctx->stream);
   launch_fill_u32(d.refine_count, 4, 0u, ctx->stream);
+  launch_fill_u32(d.defer_count, 4, 0u, ctx->stream);
   launch_fill_u32((uint32_t*)d.counters, 2 * kNumCounters, 0u, ctx->stream);
   GR_HIP(hipGetLastError());
   GR_HIP(hipStreamSynchronize(ctx->stream));
@@ -815,6 +820,7 @@ static void enqueue_step(grains_ctx* ctx) {
   hipStream_t s = ctx->stream;
   mark(ctx);
   if (d.precision == GRAINS_FP32_MIXED) launch_fill_u32(d.refine_count, 1, 0u, s);
+  if (d.split > 0) launch_fill_u32(d.defer_count, 1, 0u, s);
   launch_hash(d, s);
   mark(ctx);
   launch_sort(d, ctx->passes, &ctx->keys_s, &ctx->idx_s, s);
@@ -927,6 +933,16 @@ extern "C" int grains_set_launch(grains_ctx_t* ctx, int block_size, int narrow_b
   ctx->d.block = (uint32_t)block_size;
   ctx->d.narrow_blocks = (uint32_t)(cu_count() * narrow_blocks_per_cu);
   drop_graph(ctx);  // a captured step holds the old launch shape
+  return GRAINS_OK;
+}
+
+extern "C" int grains_set_gjk_split(grains_ctx_t* ctx, int iters) {
+  if (!ctx) return GRAINS_EINVAL;
+  if (iters == 0) iters = 4;
+  if (iters >= kGjkMaxIter) iters = -1;
+  ctx->cfg.gjk_split_iters = iters;
+  ctx->d.split = iters;
+  drop_graph(ctx);  // a captured step holds the old pass structure
   return GRAINS_OK;
 }
 

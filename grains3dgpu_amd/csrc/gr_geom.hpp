@@ -399,7 +399,11 @@ struct GjkOut {
   bool backup;    // Johnson: the backup procedure was taken; signed volumes:
                   // the search ended on rounding or the cap, not the gap test
   bool separated; // signed volumes: the lower bound passed `sep` (no contact)
+  bool deferred;  // signed volumes: stopped at a first-pass cap, undecided
 };
+
+// GJK iteration budget of a complete search
+constexpr int kGjkMaxIter = 1000;
 
 // computeClosestPoints_GJK_JH (GJK_JH.cpp:310-391), exactly the reference
 // (the REF_QUIRKS semantics): full shapes, v0 = s_A(0) - b2a(s_B(0)), the
@@ -482,6 +486,7 @@ GR_D GjkOut<T> gjk_jh(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   out.deep = false;
   out.backup = backup;
   out.separated = false;
+  out.deferred = false;
   return out;
 }
 
@@ -658,9 +663,13 @@ constexpr int kNesterovIters = 8;
 // stops with `separated` (INFINITY for a plain distance query).  Contact
 // decisions equal those of the full search, which reports no contact exactly
 // when its distance, an upper bound, is above sep -- implied by mu > sep.
+// `max_iter` < kGjkMaxIter caps a first pass: a search still undecided after
+// that many support points returns `deferred` (its result is not used; the
+// second pass repeats it from the start with the full budget, so the final
+// result is the same as one uncapped pass).
 template <class T>
 GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
-                      const GjkArgs& args, T rsum, T sep) {
+                      const GjkArgs& args, T rsum, T sep, int max_iter = kGjkMaxIter) {
   const bool nesterov = (args.flags & GRAINS_NESTEROV) != 0;
   const T rel = args.rel > 0 ? T(args.rel) : Eps<T>::LOW;
   Tr<T> b2a = relative(a2w, b2w);
@@ -680,9 +689,9 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   V3<T> v = -b2a.o, w = zero, pl = zero, ql = zero;
   T dist = norm(v), mu = T(0);
   V3<T> dir = v, wprev = v;
-  bool accel = nesterov, inside = false, ran = false, stalled = false, separated = false;
+  bool accel = nesterov, inside = false, ran = false, stalled = false, separated = false, done = false;
   int num_iter = 0;
-  while (dist > Eps<T>::HIGH && num_iter < 1000) {
+  while (dist > Eps<T>::HIGH && num_iter < max_iter) {
     V3<T> d = v;
     if (accel && num_iter >= kNesterovIters) accel = false;
     if (accel) {
@@ -701,9 +710,13 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
     if (mu < lb) mu = lb;
     if (mu > sep_s) {
       separated = true;
+      done = true;
       break;
     }
-    if (dist - mu <= dist * rel) break;
+    if (dist - mu <= dist * rel) {
+      done = true;
+      break;
+    }
     bool degen = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -714,6 +727,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
         continue;
       }
       stalled = true;
+      done = true;
       break;
     }
     // the simplex holds at most 3 points here (a tetrahedron either encloses
@@ -747,6 +761,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
     if (r.mask == 15u) {
       sv_compact(S, n, r);
       inside = true;
+      done = true;
       dist = T(0);
       break;
     }
@@ -769,6 +784,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
         continue;
       }
       stalled = true;
+      done = true;
       break;
     }
     sv_compact(S, n, r);
@@ -776,7 +792,11 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
     v = r.v;
     dist = nd;
   }
-  if (num_iter >= 1000) stalled = true;
+  // the loop ran out of support points: the iteration cap of a complete
+  // search, or the first-pass cap
+  const bool capped = !done && dist > Eps<T>::HIGH;
+  const bool deferred = capped && max_iter < kGjkMaxIter;
+  if (capped && !deferred) stalled = true;
   GjkOut<T> out;
   if (n == 0) {
     out.pa = ran ? pl : zero;
@@ -800,6 +820,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   out.deep = inside || dist <= Eps<T>::HIGH;
   out.backup = stalled;
   out.separated = separated;
+  out.deferred = deferred;
   return out;
 }
 
@@ -842,6 +863,7 @@ struct NpStats {
   int iters;
   int deep;
   int backup;
+  int deferred;   // first pass: GJK undecided at the cap (no result)
 };
 
 // closestPointsRigidBodies (CollisionDetection.cpp:181-284).  Physics
@@ -851,7 +873,7 @@ struct NpStats {
 template <class T, bool SV>
 GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<T>& a2w,
                              const Tr<T>& b2w, const GjkArgs& args, const T* verts_full,
-                             const T* verts_core, NpStats& st) {
+                             const T* verts_core, NpStats& st, int max_iter = kGjkMaxIter) {
   constexpr bool quirks = !SV;
   if (A.kind == GRAINS_SPHERE && B.kind == GRAINS_SPHERE) return cp_spheres(A.rc, B.rc, a2w, b2w);
   if (B.kind == GRAINS_RECTANGLE) return cp_rectangle(sup_ctx(A, false, verts_full, verts_core), a2w, b2w);
@@ -863,9 +885,13 @@ GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<
       const T ctSum = A.crust + B.crust;
       GjkOut<T> g;
       if constexpr (SV)
-        g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum);
+        g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum, max_iter);
       else
         g = gjk_jh(ca, cb, a2w, b2w, args);
+      if (g.deferred) {
+        st.deferred = 1;
+        return no_contact<T>();
+      }
       st.gjk = 1;
       st.iters = g.iters;
       st.backup = g.backup;

@@ -121,6 +121,9 @@ struct DevScene {
   void* pair_res;               // PairRes<T>[ecap]
   uint32_t* refine_list;        // entries to re-run in FP64 (FP32-mixed)
   uint32_t* refine_count;
+  uint32_t* defer_list;         // entries the first narrow pass left undecided
+  uint32_t* defer_count;
+  int32_t split;                // first-pass GJK cap (<= 0: one pass)
   // per particle outputs
   int32_t* contact_count;
   double *last_force, *last_torque;
@@ -178,6 +181,18 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 __device__ __forceinline__ void count_add(unsigned long long* c, uint32_t v) {
   uint32_t s = wave_sum(v);
   if (__lane_id() == 0 && s) atomicAdd(c, (unsigned long long)s);
+}
+// append `val` of every lane with `pred` to list[*count ...], one atomic per
+// wave (lanes keep their order within the wave); called by all active lanes
+__device__ __forceinline__ void wave_append(uint32_t* list, uint32_t* count, bool pred, uint32_t val) {
+  const unsigned long long mask = __ballot(pred);
+  if (!mask) return;
+  const int leader = __ffsll(mask) - 1;
+  const uint32_t lane = __lane_id();
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  if (pred) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = val;
 }
 
 // Quaternion state and pose helpers

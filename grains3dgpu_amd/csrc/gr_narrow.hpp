@@ -48,7 +48,7 @@ __device__ __forceinline__ void pair_frames(const SimGrid& g, V3<double> xi, Qt<
 // particle ids.  Returns the pair flag.
 template <class T, class R, bool SV>
 __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, uint32_t sj, PairRes<R>& out,
-                                               NpStats& st) {
+                                               NpStats& st, int max_iter = kGjkMaxIter) {
   const uint64_t n = d.n;
   const PoseRec pi = d.pk[si];
   const PoseRec pj = d.pk[sj];
@@ -61,7 +61,7 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
   pair_frames<T>(g, xi, Qt<double>{pi.qx, pi.qy, pi.qz, pi.qw}, V3<double>{pj.x, pj.y, pj.z},
                  Qt<double>{pj.qx, pj.qy, pj.qz, pj.qw}, a2w, b2w);
   const GjkArgs args = {d.flags, d.gjk_rel};
-  CInfo<T> ci = closest_points<T, SV>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
+  CInfo<T> ci = closest_points<T, SV>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st, max_iter);
   uint32_t flag = (st.gjk ? kFlagGjk : 0u) | (st.deep ? kFlagDeep : 0u);
   if (ci.overlap < T(0)) {
     flag |= kFlagContact;
@@ -97,36 +97,45 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
   return flag;
 }
 
-// One lane per candidate entry, grid-stride over the E entries of this step.
-// SV: physics semantics (signed-volume GJK); !SV: REF_QUIRKS.
-template <class T, bool SV>
+// One lane per candidate entry, grid-stride.  SV: physics semantics
+// (signed-volume GJK); !SV: REF_QUIRKS.
+// Two passes (physics semantics, d.split > 0): PASS 0 visits every entry
+// with GJK capped at d.split support points and queues the undecided ones;
+// PASS 1 runs the queued entries, now packed lane-dense, with the full
+// budget.  A wave costs as much as its slowest lane, and most entries are
+// decided in one or two iterations (separated pairs), so keeping the long
+// searches together saves the idle lanes.  The repeated search is the same
+// arithmetic from the start: results equal those of one uncapped pass.
+template <class T, bool SV, int PASS>
 __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
-  const uint32_t E = d.entry_off[d.n];
   PairRes<T>* res = (PairRes<T>*)d.pair_res;
+  const bool split = SV && PASS == 0 && d.split > 0;
+  const int cap = split ? d.split : kGjkMaxIter;
+  const uint32_t E = PASS == 0 ? d.entry_off[d.n] : *d.defer_count;
   uint32_t c_gjk = 0, c_it = 0, c_deep = 0, c_ref = 0, c_bk = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
-    NpStats st = {0, 0, 0, 0};
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride) {
+    const uint32_t e = PASS == 0 ? (uint32_t)k : d.defer_list[k];
+    NpStats st = {0, 0, 0, 0, 0};
     PairRes<T> r;
-    uint32_t flag = pair_force<T, T, SV>(d, d.pair_i[e], d.pair_j[e], r, st);
-    if (flag & kFlagContact) res[e] = r;
-    if constexpr (sizeof(T) == 4) {
+    uint32_t flag = pair_force<T, T, SV>(d, d.pair_i[e], d.pair_j[e], r, st, cap);
+    if (split) wave_append(d.defer_list, d.defer_count, st.deferred != 0, e);
+    if (!st.deferred) {
+      if (flag & kFlagContact) res[e] = r;
       // FP32-mixed: re-run in FP64 when the search ended on rounding
       // instead of the gap test, or the cores overlap (the contact normal
       // switches to the centre line there)
-      bool refine = st.backup || st.deep;
-      if (refine) {
-        flag |= kFlagRefine;
-        uint32_t slot = atomicAdd(d.refine_count, 1u);
-        d.refine_list[slot] = (uint32_t)e;
-        ++c_ref;
-      }
+      const bool refine = sizeof(T) == 4 && (st.backup || st.deep);
+      if (refine) flag |= kFlagRefine;
+      d.pair_flag[e] = flag;
+      c_gjk += st.gjk;
+      c_it += st.iters;
+      c_deep += st.deep;
+      c_bk += st.backup;
+      c_ref += refine;
     }
-    d.pair_flag[e] = flag;
-    c_gjk += st.gjk;
-    c_it += st.iters;
-    c_deep += st.deep;
-    c_bk += st.backup;
+    if constexpr (sizeof(T) == 4)
+      wave_append(d.refine_list, d.refine_count, !st.deferred && (st.backup || st.deep), e);
   }
   count_add(&d.counters[kCntGjk], c_gjk);
   count_add(&d.counters[kCntGjkIters], c_it);
