@@ -834,12 +834,16 @@ static void enqueue_step(grains_ctx* ctx) {
   launch_expand(d, s);
   mark(ctx);
   if (d.precision == GRAINS_FP64) {
-    launch_narrow64(d, s);
+    launch_narrow64(d, 0, s);
+    mark(ctx);
+    launch_narrow64(d, 1, s);
     mark(ctx);
     mark(ctx);
     launch_integrate64(d, s);
   } else {
-    launch_narrow32(d, s);
+    launch_narrow32(d, 0, s);
+    mark(ctx);
+    launch_narrow32(d, 1, s);
     mark(ctx);
     launch_refine64(d, s);
     mark(ctx);

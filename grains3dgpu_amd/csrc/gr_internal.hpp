@@ -151,8 +151,10 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* block_
 void launch_expand(const DevScene& d, hipStream_t s);
 int cu_count();
 // gr_narrow64.hip / gr_narrow32.hip
-void launch_narrow64(const DevScene& d, hipStream_t s);
-void launch_narrow32(const DevScene& d, hipStream_t s);
+// pass 0: every entry (GJK capped at d.split with the physics semantics);
+// pass 1: the entries pass 0 deferred (no launch unless split)
+void launch_narrow64(const DevScene& d, int pass, hipStream_t s);
+void launch_narrow32(const DevScene& d, int pass, hipStream_t s);
 void launch_refine64(const DevScene& d, hipStream_t s);
 void launch_integrate64(const DevScene& d, hipStream_t s);
 void launch_integrate32(const DevScene& d, hipStream_t s);

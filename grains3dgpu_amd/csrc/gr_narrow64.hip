@@ -21,13 +21,15 @@ __global__ void __launch_bounds__(kBlock) k_refine64(DevScene d) {
   }
 }
 
-void launch_narrow64(const DevScene& d, hipStream_t s) {
-  if (d.flags & GRAINS_REF_QUIRKS) {
-    hipLaunchKernelGGL((k_narrow<double, false, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
-  } else {
-    hipLaunchKernelGGL((k_narrow<double, true, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
-    if (d.split > 0)
-      hipLaunchKernelGGL((k_narrow<double, true, 1>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+void launch_narrow64(const DevScene& d, int pass, hipStream_t s) {
+  const bool sv = !(d.flags & GRAINS_REF_QUIRKS);
+  if (pass == 0) {
+    if (sv)
+      hipLaunchKernelGGL((k_narrow<double, true, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+    else
+      hipLaunchKernelGGL((k_narrow<double, false, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+  } else if (sv && d.split > 0) {
+    hipLaunchKernelGGL((k_narrow<double, true, 1>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
   }
 }
 void launch_refine64(const DevScene& d, hipStream_t s) {

@@ -182,7 +182,8 @@ class Engine:
         self._chk(self.L.grains_get_stats(self.h, C.byref(st)))
         return st.as_dict()
 
-    PHASES = ("hash", "sort", "cells", "candidates", "scan_expand", "narrow", "refine64", "integrate")
+    PHASES = ("hash", "sort", "cells", "candidates", "scan_expand", "narrow", "narrow_deferred", "refine64",
+              "integrate")
 
     def set_launch(self, block_size, narrow_blocks_per_cu):
         self._chk(self.L.grains_set_launch(self.h, block_size, narrow_blocks_per_cu))

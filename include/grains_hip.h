@@ -233,9 +233,10 @@ int grains_get_stats(grains_ctx_t* ctx, grains_stats_t* stats);
 /* Per-phase device time of the steps run while profiling is on: HIP events
  * on the context's stream around each kernel group, summed over the steps
  * of the last grains_step call.  Phases, in order: cell hash, radix sort,
- * cell ranges, candidates, scan + expand, narrow phase + force, FP64
- * re-run, reduction + walls + integration. */
-#define GRAINS_NUM_PHASES 8
+ * cell ranges + sorted records, candidates, scan + expand, narrow phase +
+ * force (first pass), narrow phase of the deferred entries (second pass),
+ * FP64 re-run, reduction + walls + integration. */
+#define GRAINS_NUM_PHASES 9
 int grains_set_profiling(grains_ctx_t* ctx, int on);
 int grains_get_phase_times(grains_ctx_t* ctx, double* ms, int n_phases);
 
