@@ -76,6 +76,25 @@ def build_cpp_tests(verbose=False):
     return outs
 
 
+def build_adjoint_check(verbose=False):
+    """Host library of the step's vector-Jacobian products
+    (tests/cpp/adjoint_check.cc over csrc/gr_adjoint.hpp) for the CPU tests."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "adjoint_check.cc")
+    out_dir = os.path.join(root, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libadjoint_check.so")
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    cmd = [hipcc, "-x", "hip", f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-fPIC", "-shared",
+           "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", src, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on adjoint_check.cc:\n{r.stderr[-3000:]}")
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "gr_internal.hpp"
+#include "gr_move.hpp"
 
 namespace gr {
 
@@ -228,36 +229,9 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
     store3(d.last_torque, n, i, M);
     d.contact_count[i] += ncon;
     contacts = (uint32_t)ncon;
-    // RigidBody::computeMomentum (RigidBody.cpp:267-307) with the quaternion state
-    const Qt<double> qc = conj(qi);
-    const V3<double> av = mult_to_v3(qc, vq(om, qi));
-    V3<double> am = mult_to_v3(qc, vq(M, qi));
-    const double* I = dy.inertia;
-    const double* Ii = dy.inv;
-    V3<double> tmp = {I[0] * av.x + I[1] * av.y + I[2] * av.z, I[1] * av.x + I[3] * av.y + I[4] * av.z,
-                      I[2] * av.x + I[4] * av.y + I[5] * av.z};
-    am = am + cross(tmp, av);
-    tmp = {Ii[0] * am.x + Ii[1] * am.y + Ii[2] * am.z, Ii[1] * am.x + Ii[3] * am.y + Ii[4] * am.z,
-           Ii[2] * am.x + Ii[4] * am.y + Ii[5] * am.z};
-    const V3<double> alpha = mult_to_v3(qi, vq(tmp, qc));
-    const V3<double> accel = F / dy.mass;
-    // FirstOrderExplicit::Move (FirstOrderExplicit.cpp:46-60)
-    const double dt = d.dt;
-    const V3<double> trans = dt * vel;
-    const V3<double> vnew = vel + dt * accel;
-    Qt<double> dq = {0.0, 0.0, 0.0, 1.0};
-    const double nO = norm(om);
-    if (nO > Eps<double>::HIGH) {
-      // TimeIntegrator::computeQuaternionChange (TimeIntegrator.cpp:28-44)
-      const double c = cos(nO * dt / 2.0);
-      const double sn = sin(nO * dt / 2.0);
-      dq = qmake((sn / nO) * om, c);
-    }
-    const V3<double> onew = om + dt * alpha;
-    // Transform3::updateTransform (Transform3.cpp:323-330): translate, then
-    // rotate on the left by dq -- on the quaternion: q <- dq x q
-    V3<double> x = xi + trans;
-    const Qt<double> qn = qmul(dq, qi);
+    V3<double> x = xi, vnew = vel, onew = om;
+    Qt<double> qn = qi;
+    move_particle(dy.mass, dy.inertia, dy.inv, d.dt, F, M, x, qn, vnew, onew);
     if (g.periodic) {
       // periodic wrap of the centre (not in the reference)
       auto wrap = [&](double c, int k) {

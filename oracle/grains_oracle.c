@@ -1805,11 +1805,11 @@ double go_gjk(const grains_shape_t* a, const grains_shape_t* b, const double* ve
 }
 
 int go_closest_points(const grains_shape_t* a, const grains_shape_t* b, const double* vertices,
-                      const double* ta, const double* tb, uint32_t flags, double* out) {
+                      const double* ta, const double* tb, uint32_t flags, double rel_tol, double* out) {
   oshape A, B;
   if (shape_derive(&A, a, vertices) || shape_derive(&B, b, vertices)) return -1;
   tr3 TA = tr_from12(ta), TB = tr_from12(tb);
-  cinfo c = closest_points(&A, &B, &TA, &TB, flags, 0.0, NULL);
+  cinfo c = closest_points(&A, &B, &TA, &TB, flags, rel_tol, NULL);
   out[0] = c.cp.x; out[1] = c.cp.y; out[2] = c.cp.z;
   out[3] = c.cv.x; out[4] = c.cv.y; out[5] = c.cv.z;
   out[6] = c.overlap;

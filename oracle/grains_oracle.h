@@ -83,7 +83,7 @@ double go_gjk(const grains_shape_t* a, const grains_shape_t* b,
  * vector[3], overlap.  Returns 1 on contact. */
 int go_closest_points(const grains_shape_t* a, const grains_shape_t* b,
                       const double* vertices, const double* ta,
-                      const double* tb, uint32_t flags, double* out);
+                      const double* tb, uint32_t flags, double rel_tol, double* out);
 /* Hooke force on A for one contact (HookeContactForceModel.cpp:91-186).
  * ci = cp[3], cv[3], overlap; out = force[3], torque[3]. */
 void go_hooke(const grains_hooke_t* p, const double* ci, const double* rel_vel,

@@ -63,7 +63,7 @@ def lib():
         L.go_gjk.argtypes = [C.POINTER(A.Shape), C.POINTER(A.Shape), vp, vp, vp, C.c_uint32,
                              C.c_double, vp, vp, C.POINTER(C.c_int)]
         L.go_closest_points.argtypes = [C.POINTER(A.Shape), C.POINTER(A.Shape), vp, vp, vp,
-                                        C.c_uint32, vp]
+                                        C.c_uint32, C.c_double, vp]
         L.go_hooke.argtypes = [C.POINTER(A.Hooke), vp, vp, vp, C.c_double, C.c_double, vp, vp]
         L.go_quat_from_matrix.argtypes = [vp, vp]
         L.go_matrix_from_quat.argtypes = [vp, vp]
@@ -253,11 +253,11 @@ def gjk(sa, sb, ta, tb, flags=0, rel_tol=0.0, vertices=None):
     return d, pa, pb, it.value
 
 
-def closest_points(sa, sb, ta, tb, flags=0, vertices=None):
+def closest_points(sa, sb, ta, tb, flags=0, vertices=None, rel_tol=0.0):
     out = np.zeros(7)
     v = _c(vertices)
     hit = lib().go_closest_points(C.byref(_shape(sa)), C.byref(_shape(sb)), A.dptr(v),
-                                  A.dptr(_c(ta)), A.dptr(_c(tb)), flags, A.dptr(out))
+                                  A.dptr(_c(ta)), A.dptr(_c(tb)), flags, C.c_double(rel_tol), A.dptr(out))
     return bool(hit), out[:3].copy(), out[3:6].copy(), out[6]
 
 
