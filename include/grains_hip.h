@@ -126,8 +126,9 @@ typedef struct grains_config_t {
   int32_t gjk_split_iters;  /* physics semantics: the narrow phase runs in two
                                passes -- every entry with GJK capped at this
                                many iterations, then the undecided entries,
-                               compacted, with the full budget (0: 4;
-                               < 0: one pass).  Results do not depend on it. */
+                               compacted, with the full budget (0: 4 in
+                               FP64, one pass in FP32-mixed; < 0: one pass).
+                               Results do not depend on it.                 */
   int32_t reserved[2];
 } grains_config_t;
 
