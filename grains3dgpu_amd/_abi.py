@@ -41,7 +41,7 @@ class Config(C.Structure):
     _fields_ = [("dt", C.c_double), ("precision", C.c_int32),
                 ("flags", C.c_uint32), ("gjk_rel_tol", C.c_double),
                 ("max_neighbors", C.c_int32), ("block_size", C.c_int32),
-                ("narrow_blocks_per_cu", C.c_int32), ("gjk_split_iters", C.c_int32),
+                ("narrow_blocks_per_cu", C.c_int32), ("narrow_passes", C.c_int32),
                 ("reserved", C.c_int32 * 2)]
 
 

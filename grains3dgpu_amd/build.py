@@ -26,10 +26,16 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
           "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 
 
+# the FP32-mixed unit: FP32 division and square root need not be correctly
+# rounded (<= 2.5 ulp, no IEEE division sequence); its results are checked
+# against the FP64 oracle to tolerances, and FP64 code is unaffected
+EXTRA = {"gr_narrow32": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+
+
 def _compile(unit, contract, hipcc):
     src = os.path.join(CSRC, unit + ".hip")
     obj = os.path.join(BUILD, unit + ".o")
-    cmd = [hipcc] + COMMON + [f"-ffp-contract={contract}", "-c", src, "-o", obj]
+    cmd = [hipcc] + COMMON + [f"-ffp-contract={contract}"] + EXTRA.get(unit, []) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {unit}:\n{r.stderr[-4000:]}")

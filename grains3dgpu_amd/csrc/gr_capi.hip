@@ -383,10 +383,11 @@ extern "C" int grains_create(const grains_config_t* cfg, grains_ctx_t** out) {
     return GRAINS_EINVAL;
   }
   if (ctx->cfg.narrow_blocks_per_cu <= 0) ctx->cfg.narrow_blocks_per_cu = 8;
-  // measured defaults: two passes pay in FP64 (3.8 -> 2.4 ms of narrow phase
-  // on the resting superquadric bed at 1e6 particles), not in FP32
-  if (ctx->cfg.gjk_split_iters == 0) ctx->cfg.gjk_split_iters = ctx->cfg.precision == GRAINS_FP64 ? 4 : -1;
-  if (ctx->cfg.gjk_split_iters >= kGjkMaxIter) ctx->cfg.gjk_split_iters = -1;
+  if (ctx->cfg.narrow_passes == 0) ctx->cfg.narrow_passes = 2;
+  if (ctx->cfg.narrow_passes != 1 && ctx->cfg.narrow_passes != 2) {
+    delete ctx;
+    return GRAINS_EINVAL;
+  }
   std::memset(&ctx->stats, 0, sizeof(ctx->stats));
   std::memset(&ctx->d, 0, sizeof(ctx->d));
   int ndev = 0;
@@ -696,7 +697,7 @@ static int finalize(grains_ctx* ctx) {
   d.precision = ctx->cfg.precision;
   d.block = (uint32_t)ctx->cfg.block_size;
   d.narrow_blocks = (uint32_t)(cu_count() * ctx->cfg.narrow_blocks_per_cu);
-  d.split = ctx->cfg.gjk_split_iters;
+  d.split = ctx->cfg.narrow_passes == 2 ? 1 : 0;
   launch_fill_u32(d.cand_count, n, 0u, ctx->stream);
   launch_fill_u32(d.entry_off, n + 1, 0u, ctx->stream);
   launch_fill_u32(d.refine_count, 4, 0u, ctx->stream);
@@ -942,12 +943,10 @@ extern "C" int grains_set_launch(grains_ctx_t* ctx, int block_size, int narrow_b
   return GRAINS_OK;
 }
 
-extern "C" int grains_set_gjk_split(grains_ctx_t* ctx, int iters) {
-  if (!ctx) return GRAINS_EINVAL;
-  if (iters == 0) iters = ctx->cfg.precision == GRAINS_FP64 ? 4 : -1;
-  if (iters >= kGjkMaxIter) iters = -1;
-  ctx->cfg.gjk_split_iters = iters;
-  ctx->d.split = iters;
+extern "C" int grains_set_narrow_passes(grains_ctx_t* ctx, int passes) {
+  if (!ctx || (passes != 1 && passes != 2)) return GRAINS_EINVAL;
+  ctx->cfg.narrow_passes = passes;
+  ctx->d.split = passes == 2 ? 1 : 0;
   drop_graph(ctx);  // a captured step holds the old pass structure
   return GRAINS_OK;
 }

@@ -399,7 +399,7 @@ struct GjkOut {
   bool backup;    // Johnson: the backup procedure was taken; signed volumes:
                   // the search ended on rounding or the cap, not the gap test
   bool separated; // signed volumes: the lower bound passed `sep` (no contact)
-  bool deferred;  // signed volumes: stopped at a first-pass cap, undecided
+  bool deferred;  // screening search: not separated by the first support pair
 };
 
 // GJK iteration budget of a complete search
@@ -663,13 +663,14 @@ constexpr int kNesterovIters = 8;
 // stops with `separated` (INFINITY for a plain distance query).  Contact
 // decisions equal those of the full search, which reports no contact exactly
 // when its distance, an upper bound, is above sep -- implied by mu > sep.
-// `max_iter` < kGjkMaxIter caps a first pass: a search still undecided after
-// that many support points returns `deferred` (its result is not used; the
-// second pass repeats it from the start with the full budget, so the final
-// result is the same as one uncapped pass).
-template <class T>
+// SCREEN: stop after the first support pair's separation test; a pair it
+// does not separate returns `deferred` (no result: the full search repeats it
+// from the start).  Everything up to that test is this same code, so the
+// screen separates exactly the pairs the full search separates at its first
+// support pair.
+template <class T, bool SCREEN = false>
 GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
-                      const GjkArgs& args, T rsum, T sep, int max_iter = kGjkMaxIter) {
+                      const GjkArgs& args, T rsum, T sep) {
   const bool nesterov = (args.flags & GRAINS_NESTEROV) != 0;
   const T rel = args.rel > 0 ? T(args.rel) : Eps<T>::LOW;
   Tr<T> b2a = relative(a2w, b2w);
@@ -691,7 +692,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   V3<T> dir = v, wprev = v;
   bool accel = nesterov, inside = false, ran = false, stalled = false, separated = false, done = false;
   int num_iter = 0;
-  while (dist > Eps<T>::HIGH && num_iter < max_iter) {
+  while (dist > Eps<T>::HIGH && num_iter < kGjkMaxIter) {
     V3<T> d = v;
     if (accel && num_iter >= kNesterovIters) accel = false;
     if (accel) {
@@ -713,6 +714,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
       done = true;
       break;
     }
+    if constexpr (SCREEN) break;
     if (dist - mu <= dist * rel) {
       done = true;
       break;
@@ -792,11 +794,9 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
     v = r.v;
     dist = nd;
   }
-  // the loop ran out of support points: the iteration cap of a complete
-  // search, or the first-pass cap
-  const bool capped = !done && dist > Eps<T>::HIGH;
-  const bool deferred = capped && max_iter < kGjkMaxIter;
-  if (capped && !deferred) stalled = true;
+  // the loop ran out of support points (iteration cap)
+  if (!SCREEN && !done && dist > Eps<T>::HIGH) stalled = true;
+  const bool deferred = SCREEN && !separated;
   GjkOut<T> out;
   if (n == 0) {
     out.pa = ran ? pl : zero;
@@ -863,20 +863,29 @@ struct NpStats {
   int iters;
   int deep;
   int backup;
-  int deferred;   // first pass: GJK undecided at the cap (no result)
+  int deferred;   // screening pass: not decided there (no result)
 };
 
 // closestPointsRigidBodies (CollisionDetection.cpp:181-284).  Physics
 // semantics run GJK on the crust cores (the commented intent of lines 231-236).
 // SV selects the physics semantics (signed-volume GJK on the crust cores);
 // !SV is REF_QUIRKS (Johnson's sub-algorithm on the full shapes).
-template <class T, bool SV>
+// SCREEN (physics semantics): decide only the pairs without contact that the
+// bounding sphere, the OBB or GJK's first support pair settle; every other
+// pair returns st.deferred for the full pass.
+template <class T, bool SV, bool SCREEN = false>
 GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<T>& a2w,
                              const Tr<T>& b2w, const GjkArgs& args, const T* verts_full,
-                             const T* verts_core, NpStats& st, int max_iter = kGjkMaxIter) {
+                             const T* verts_core, NpStats& st) {
   constexpr bool quirks = !SV;
-  if (A.kind == GRAINS_SPHERE && B.kind == GRAINS_SPHERE) return cp_spheres(A.rc, B.rc, a2w, b2w);
-  if (B.kind == GRAINS_RECTANGLE) return cp_rectangle(sup_ctx(A, false, verts_full, verts_core), a2w, b2w);
+  if ((A.kind == GRAINS_SPHERE && B.kind == GRAINS_SPHERE) || B.kind == GRAINS_RECTANGLE) {
+    if constexpr (SCREEN) {
+      st.deferred = 1;
+      return no_contact<T>();
+    }
+    if (B.kind == GRAINS_RECTANGLE) return cp_rectangle(sup_ctx(A, false, verts_full, verts_core), a2w, b2w);
+    return cp_spheres(A.rc, B.rc, a2w, b2w);
+  }
   T overlap = norm(b2w.o - a2w.o) - (A.rc + B.rc);
   if (overlap < T(0)) {
     if (obb_intersect(A.bbox, B.bbox, a2w, b2w)) {
@@ -885,7 +894,7 @@ GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<
       const T ctSum = A.crust + B.crust;
       GjkOut<T> g;
       if constexpr (SV)
-        g = gjk_sv(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum, max_iter);
+        g = gjk_sv<T, SCREEN>(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum);
       else
         g = gjk_jh(ca, cb, a2w, b2w, args);
       if (g.deferred) {

@@ -123,7 +123,7 @@ struct DevScene {
   uint32_t* refine_count;
   uint32_t* defer_list;         // entries the first narrow pass left undecided
   uint32_t* defer_count;
-  int32_t split;                // first-pass GJK cap (<= 0: one pass)
+  int32_t split;                // 1: screening pass + pass over the queued entries
   // per particle outputs
   int32_t* contact_count;
   double *last_force, *last_torque;
@@ -151,8 +151,8 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* block_
 void launch_expand(const DevScene& d, hipStream_t s);
 int cu_count();
 // gr_narrow64.hip / gr_narrow32.hip
-// pass 0: every entry (GJK capped at d.split with the physics semantics);
-// pass 1: the entries pass 0 deferred (no launch unless split)
+// pass 0: every entry (complete, or screened with d.split);
+// pass 1: the entries the screen queued (no launch unless d.split)
 void launch_narrow64(const DevScene& d, int pass, hipStream_t s);
 void launch_narrow32(const DevScene& d, int pass, hipStream_t s);
 void launch_refine64(const DevScene& d, hipStream_t s);

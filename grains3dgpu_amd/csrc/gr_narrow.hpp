@@ -47,9 +47,9 @@ __device__ __forceinline__ void pair_frames(const SimGrid& g, V3<double> xi, Qt<
 // si's side (ComponentManagerCPU.cpp:384-419).  Poses come from the sorted
 // records; velocities, needed only for contacts, from the planes through the
 // particle ids.  Returns the pair flag.
-template <class T, class R, bool SV>
+template <class T, class R, bool SV, bool SCREEN = false>
 __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, uint32_t sj, PairRes<R>& out,
-                                               NpStats& st, int max_iter = kGjkMaxIter) {
+                                               NpStats& st) {
   const uint64_t n = d.n;
   const PoseRec pi = d.pk[si];
   const PoseRec pj = d.pk[sj];
@@ -62,8 +62,9 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
   pair_frames<T>(g, xi, Qt<double>{pi.qx, pi.qy, pi.qz, pi.qw}, V3<double>{pj.x, pj.y, pj.z},
                  Qt<double>{pj.qx, pj.qy, pj.qz, pj.qw}, a2w, b2w);
   const GjkArgs args = {d.flags, d.gjk_rel};
-  CInfo<T> ci = closest_points<T, SV>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st, max_iter);
+  CInfo<T> ci = closest_points<T, SV, SCREEN>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
   uint32_t flag = (st.gjk ? kFlagGjk : 0u) | (st.deep ? kFlagDeep : 0u);
+  if constexpr (SCREEN) return flag;  // decided pairs have no contact
   if (ci.overlap < T(0)) {
     flag |= kFlagContact;
     const uint32_t i = d.sidx[si], j = d.sidx[sj];
@@ -100,27 +101,27 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
 
 // One lane per candidate entry, grid-stride.  SV: physics semantics
 // (signed-volume GJK); !SV: REF_QUIRKS.
-// Two passes (physics semantics, d.split > 0): PASS 0 visits every entry
-// with GJK capped at d.split support points and queues the undecided ones;
-// PASS 1 runs the queued entries, now packed lane-dense, with the full
-// budget.  A wave costs as much as its slowest lane, and most entries are
-// decided in one or two iterations (separated pairs), so keeping the long
-// searches together saves the idle lanes.  The repeated search is the same
-// arithmetic from the start: results equal those of one uncapped pass.
+// PASS 0: every entry, complete.  With two passes (physics semantics):
+// PASS 2 screens every entry -- bounding sphere, OBB and GJK's first support
+// pair decide most pairs of a resting or loose packing (no contact) with a
+// small kernel -- and queues the rest; PASS 1 runs the queued entries, packed
+// lane-dense, complete.  A wave costs as much as its slowest lane, so the
+// long searches are kept together and the screen's waves stay short.  The
+// queued searches start from scratch with the same arithmetic: results equal
+// those of PASS 0 bit for bit.
 template <class T, bool SV, int PASS>
 __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
+  constexpr bool screen = SV && PASS == 2;
   PairRes<T>* res = (PairRes<T>*)d.pair_res;
-  const bool split = SV && PASS == 0 && d.split > 0;
-  const int cap = split ? d.split : kGjkMaxIter;
-  const uint32_t E = PASS == 0 ? d.entry_off[d.n] : *d.defer_count;
+  const uint32_t E = PASS == 1 ? *d.defer_count : d.entry_off[d.n];
   uint32_t c_gjk = 0, c_it = 0, c_deep = 0, c_ref = 0, c_bk = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride) {
-    const uint32_t e = PASS == 0 ? (uint32_t)k : d.defer_list[k];
+    const uint32_t e = PASS == 1 ? d.defer_list[k] : (uint32_t)k;
     NpStats st = {0, 0, 0, 0, 0};
     PairRes<T> r;
-    uint32_t flag = pair_force<T, T, SV>(d, d.pair_i[e], d.pair_j[e], r, st, cap);
-    if (split) wave_append(d.defer_list, d.defer_count, st.deferred != 0, e);
+    uint32_t flag = pair_force<T, T, SV, screen>(d, d.pair_i[e], d.pair_j[e], r, st);
+    if constexpr (screen) wave_append(d.defer_list, d.defer_count, st.deferred != 0, e);
     if (!st.deferred) {
       if (flag & kFlagContact) res[e] = r;
       // FP32-mixed: re-run in FP64 when the search ended on rounding
@@ -135,7 +136,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
       c_bk += st.backup;
       c_ref += refine;
     }
-    if constexpr (sizeof(T) == 4)
+    if constexpr (sizeof(T) == 4 && !screen)
       wave_append(d.refine_list, d.refine_count, !st.deferred && (st.backup || st.deep), e);
   }
   count_add(&d.counters[kCntGjk], c_gjk);

@@ -6,12 +6,15 @@ namespace gr {
 
 void launch_narrow32(const DevScene& d, int pass, hipStream_t s) {
   const bool sv = !(d.flags & GRAINS_REF_QUIRKS);
+  const bool two = sv && d.split > 0;
   if (pass == 0) {
-    if (sv)
+    if (two)
+      hipLaunchKernelGGL((k_narrow<float, true, 2>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+    else if (sv)
       hipLaunchKernelGGL((k_narrow<float, true, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
     else
       hipLaunchKernelGGL((k_narrow<float, false, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
-  } else if (sv && d.split > 0) {
+  } else if (two) {
     hipLaunchKernelGGL((k_narrow<float, true, 1>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
   }
 }

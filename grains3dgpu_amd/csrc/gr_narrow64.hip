@@ -23,12 +23,15 @@ __global__ void __launch_bounds__(kBlock) k_refine64(DevScene d) {
 
 void launch_narrow64(const DevScene& d, int pass, hipStream_t s) {
   const bool sv = !(d.flags & GRAINS_REF_QUIRKS);
+  const bool two = sv && d.split > 0;
   if (pass == 0) {
-    if (sv)
+    if (two)
+      hipLaunchKernelGGL((k_narrow<double, true, 2>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+    else if (sv)
       hipLaunchKernelGGL((k_narrow<double, true, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
     else
       hipLaunchKernelGGL((k_narrow<double, false, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
-  } else if (sv && d.split > 0) {
+  } else if (two) {
     hipLaunchKernelGGL((k_narrow<double, true, 1>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
   }
 }

@@ -52,7 +52,7 @@ def lib():
                                               C.POINTER(C.c_float)]
         L.grains_set_profiling.argtypes = [vp, C.c_int]
         L.grains_set_launch.argtypes = [vp, C.c_int, C.c_int]
-        L.grains_set_gjk_split.argtypes = [vp, C.c_int]
+        L.grains_set_narrow_passes.argtypes = [vp, C.c_int]
         L.grains_get_phase_times.argtypes = [vp, vp, C.c_int]
         L.grains_snapshot.argtypes = [vp, vp, C.POINTER(C.c_size_t)]
         L.grains_restore.argtypes = [vp, vp, C.c_size_t]
@@ -72,13 +72,13 @@ class Engine:
     """One grains_ctx_t.  precision: _abi.FP64 | _abi.FP32_MIXED."""
 
     def __init__(self, dt=1e-6, precision=A.FP64, flags=0, gjk_rel_tol=0.0, max_neighbors=0,
-                 block_size=0, narrow_blocks_per_cu=0, gjk_split_iters=0):
+                 block_size=0, narrow_blocks_per_cu=0, narrow_passes=0):
         self.L = lib()
         cfg = A.Config()
         cfg.dt, cfg.precision, cfg.flags, cfg.gjk_rel_tol = dt, precision, flags, gjk_rel_tol
         cfg.max_neighbors, cfg.block_size = max_neighbors, block_size
         cfg.narrow_blocks_per_cu = narrow_blocks_per_cu
-        cfg.gjk_split_iters = gjk_split_iters
+        cfg.narrow_passes = narrow_passes
         h = vp()
         rc = self.L.grains_create(C.byref(cfg), C.byref(h))
         if rc != 0:
@@ -188,9 +188,9 @@ class Engine:
     def set_launch(self, block_size, narrow_blocks_per_cu):
         self._chk(self.L.grains_set_launch(self.h, block_size, narrow_blocks_per_cu))
 
-    def set_gjk_split(self, iters):
-        """First-pass GJK iteration cap of the two-pass narrow phase (< 0: one pass)."""
-        self._chk(self.L.grains_set_gjk_split(self.h, iters))
+    def set_narrow_passes(self, passes):
+        """1: one complete narrow pass; 2: screening pass + pass over the queued entries."""
+        self._chk(self.L.grains_set_narrow_passes(self.h, passes))
 
     def set_profiling(self, on=True):
         self._chk(self.L.grains_set_profiling(self.h, 1 if on else 0))

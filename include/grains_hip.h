@@ -123,12 +123,11 @@ typedef struct grains_config_t {
                                per-entry kernels: 64, 128 or 256 (0: 256)    */
   int32_t narrow_blocks_per_cu; /* resident blocks per CU of the persistent
                                narrow-phase launch (0: 8)                   */
-  int32_t gjk_split_iters;  /* physics semantics: the narrow phase runs in two
-                               passes -- every entry with GJK capped at this
-                               many iterations, then the undecided entries,
-                               compacted, with the full budget (0: 4 in
-                               FP64, one pass in FP32-mixed; < 0: one pass).
-                               Results do not depend on it.                 */
+  int32_t narrow_passes;    /* physics semantics: 2 = a screening pass over
+                               every entry (bounding sphere, OBB, GJK's first
+                               support pair) and a complete pass over the
+                               undecided entries, compacted; 1 = one complete
+                               pass (0: 2).  Results do not depend on it.   */
   int32_t reserved[2];
 } grains_config_t;
 
@@ -199,11 +198,10 @@ int grains_get_transforms(grains_ctx_t* ctx, double* transforms, uint64_t n);
  * autotuner.  No counterpart in the reference (its kernels hard-code 256,
  * ComponentManagerGPU.cpp:265-352). */
 int grains_set_launch(grains_ctx_t* ctx, int block_size, int narrow_blocks_per_cu);
-/* First-pass GJK iteration cap of the two-pass narrow phase (see
- * grains_config_t.gjk_split_iters), changeable between steps; used by the
- * autotuner.  No counterpart in the reference (one pass, one lane per pair,
- * ComponentManagerGPU_Kernels.cpp:141-226). */
-int grains_set_gjk_split(grains_ctx_t* ctx, int iters);
+/* Passes of the narrow phase (1 or 2, see grains_config_t.narrow_passes),
+ * changeable between steps; used by the autotuner.  No counterpart in the
+ * reference (one pass, one lane per pair, ComponentManagerGPU_Kernels.cpp:141-226). */
+int grains_set_narrow_passes(grains_ctx_t* ctx, int passes);
 
 /* Time stepping ------------------------------------------------------------ */
 /* n_steps iterations of the GrainsCPU::simulate loop body

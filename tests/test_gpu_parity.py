@@ -271,20 +271,20 @@ def test_gjk_batch_matches_oracle():
 
 @pytest.mark.parametrize("precision", [A.FP64, A.FP32_MIXED])
 def test_two_pass_narrow_phase_equals_one_pass(precision):
-    """The first narrow pass caps GJK and queues the undecided entries; the
-    second repeats them from the start.  Same arithmetic, so the state,
-    forces and counters equal one uncapped pass bit for bit, whatever the
-    cap (1 defers nearly every search, 4 is the default)."""
+    """The screening pass decides the pairs that the bounding sphere, the
+    OBB or GJK's first support pair settle and queues the rest; the second
+    pass repeats those from the start.  Same arithmetic, so the state, forces
+    and counters equal one complete pass bit for bit."""
     from grains3dgpu_amd import engine
     sc = dense_superquadrics(n=300, seed=81, phi=0.12, vel=0.2)
     out = {}
-    for split in (-1, 1, 4):
-        e = S.load(engine.Engine(dt=sc.dt, precision=precision, gjk_rel_tol=1e-8, gjk_split_iters=split), sc)
+    for split in (1, 2):
+        e = S.load(engine.Engine(dt=sc.dt, precision=precision, gjk_rel_tol=1e-8, narrow_passes=split), sc)
         e.step(20)
         out[split] = (e.get_state(), e.get_torce(), e.get_stats(), e.get_pairs(A.PAIRS_CONTACT))
-    ref = out[-1]
+    ref = out[1]
     assert ref[2]["contacts"] > 0 and ref[2]["gjk_calls"] > 0
-    for split in (1, 4):
+    for split in (2,):
         for a, b in zip(out[split][0] + out[split][1], ref[0] + ref[1]):
             np.testing.assert_array_equal(a, b)
         np.testing.assert_array_equal(out[split][3], ref[3])
