@@ -53,7 +53,7 @@ __global__ void k_hash(uint64_t n, const double* __restrict__ pos, const uint32_
     }
     key[i] = g.base + (c[2] * g.n3[1] + c[1]) * g.n3[0] + c[0];
   }
-  count_add(&counters[kCntOod], ood);
+  count_add(counters, kCntOod, ood);
 }
 
 void launch_hash(const DevScene& d, hipStream_t s) {
@@ -330,9 +330,8 @@ __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_
     }
     d.cand_count[s] = cnt < K ? cnt : K;
   }
-  count_add(&d.counters[kCntOverflow], over);
-  uint32_t m = wave_max(cnt);
-  if (__lane_id() == 0 && m) atomicMax((unsigned long long*)&d.counters[kCntMaxCand], (unsigned long long)m);
+  count_add(d.counters, kCntOverflow, over);
+  count_max(d.counters, kCntMaxCand, cnt);
 }
 
 void launch_candidates(const DevScene& d, const uint32_t* keys_s, hipStream_t s) {
@@ -355,7 +354,7 @@ __global__ void k_expand(uint64_t n, const uint32_t* __restrict__ cand, const ui
       pj[e0 + k] = cand[(uint64_t)k * n + s];
     }
   }
-  count_add(&counters[kCntCandidates], c);
+  count_add(counters, kCntCandidates, c);
 }
 
 void launch_expand(const DevScene& d, hipStream_t s) {

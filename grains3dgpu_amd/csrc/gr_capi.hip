@@ -519,6 +519,16 @@ extern "C" int grains_set_particles(grains_ctx_t* ctx, const uint32_t* shape, co
 }
 
 // cell layout, obstacle ranges, shape tables and work buffers
+// Segment of the screening pass's queue per block: a block of the persistent
+// narrow launch visits at most ceil(ecap / (blocks * threads)) * threads
+// entries, so blocks * defer_cap <= ecap + blocks * threads, within the
+// allocation (ecap + 64 blocks per CU * 256 threads).
+static void set_defer_cap(grains_ctx* ctx) {
+  DevScene& d = ctx->d;
+  const uint64_t per = (uint64_t)d.narrow_blocks * d.block;
+  d.defer_cap = (uint32_t)(((d.ecap + per - 1) / per) * d.block);
+}
+
 static int finalize(grains_ctx* ctx) {
   if (ctx->layout_ok) return GRAINS_OK;
   if (!ctx->n) return fail(ctx, GRAINS_ESTATE, "no particles");
@@ -626,7 +636,11 @@ static int finalize(grains_ctx* ctx) {
   const uint32_t K = (uint32_t)ctx->cfg.max_neighbors;
   const uint64_t ecap = n * K;
   const uint64_t ntiles = (n + kTile - 1) / kTile;
-  const uint64_t nblk = std::max((n + kTile - 1) / kTile, (256 * ntiles + kTile - 1) / kTile) + 2;
+  // the narrow phase's launch may hold up to 64 blocks per CU of up to 256
+  // threads (grains_set_launch); the screen's queue has one segment per block
+  const uint64_t max_nb = (uint64_t)cu_count() * 64;
+  const uint64_t nblk =
+      std::max(std::max((n + kTile - 1) / kTile, (256 * ntiles + kTile - 1) / kTile), (max_nb + kTile) / kTile) + 2;
   const size_t res_sz = ctx->cfg.precision == GRAINS_FP64 ? sizeof(PairRes<double>) : sizeof(PairRes<float>);
   int rc;
   ShapeNP<double>* p64;
@@ -655,8 +669,9 @@ static int finalize(grains_ctx* ctx) {
       (rc = dalloc(ctx, &d.entry_off, (n + 1) * 4, L)) || (rc = dalloc(ctx, &d.pair_i, ecap * 4, L)) ||
       (rc = dalloc(ctx, &d.pair_j, ecap * 4, L)) || (rc = dalloc(ctx, &d.pair_flag, ecap * 4, L)) ||
       (rc = dalloc(ctx, &d.pair_res, ecap * res_sz, L)) || (rc = dalloc(ctx, &d.refine_list, ecap * 4, L)) ||
-      (rc = dalloc(ctx, &d.refine_count, 16, L)) || (rc = dalloc(ctx, &d.defer_list, ecap * 4, L)) ||
-      (rc = dalloc(ctx, &d.defer_count, 16, L)) || (rc = dalloc(ctx, &d.counters, kNumCounters * 8, L)))
+      (rc = dalloc(ctx, &d.refine_count, 16, L)) || (rc = dalloc(ctx, &d.defer_list, (ecap + max_nb * 256) * 4, L)) ||
+      (rc = dalloc(ctx, &d.defer_count, max_nb * 4, L)) || (rc = dalloc(ctx, &d.defer_off, (max_nb + 1) * 4, L)) ||
+      (rc = dalloc(ctx, &d.counters, (size_t)kNumCounters * kCntSlots * 8, L)))
     return rc;
   GR_HIP(hipMemcpy(p64, np64.data(), S * sizeof(ShapeNP<double>), hipMemcpyHostToDevice));
   GR_HIP(hipMemcpy(p32, np32.data(), S * sizeof(ShapeNP<float>), hipMemcpyHostToDevice));
@@ -701,8 +716,9 @@ static int finalize(grains_ctx* ctx) {
   launch_fill_u32(d.cand_count, n, 0u, ctx->stream);
   launch_fill_u32(d.entry_off, n + 1, 0u, ctx->stream);
   launch_fill_u32(d.refine_count, 4, 0u, ctx->stream);
-  launch_fill_u32(d.defer_count, 4, 0u, ctx->stream);
-  launch_fill_u32((uint32_t*)d.counters, 2 * kNumCounters, 0u, ctx->stream);
+  launch_fill_u32(d.defer_count, max_nb, 0u, ctx->stream);
+  set_defer_cap(ctx);
+  launch_fill_u32((uint32_t*)d.counters, 2 * kNumCounters * kCntSlots, 0u, ctx->stream);
   GR_HIP(hipGetLastError());
   GR_HIP(hipStreamSynchronize(ctx->stream));
   ctx->layout_ok = true;
@@ -823,7 +839,6 @@ static void enqueue_step(grains_ctx* ctx) {
   hipStream_t s = ctx->stream;
   mark(ctx);
   if (d.precision == GRAINS_FP32_MIXED) launch_fill_u32(d.refine_count, 1, 0u, s);
-  if (d.split > 0) launch_fill_u32(d.defer_count, 1, 0u, s);
   launch_hash(d, s);
   mark(ctx);
   launch_sort(d, ctx->passes, &ctx->keys_s, &ctx->idx_s, s);
@@ -856,8 +871,16 @@ static void enqueue_step(grains_ctx* ctx) {
 }
 
 static int read_stats(grains_ctx* ctx, uint32_t steps) {
+  std::vector<unsigned long long> slots((size_t)kNumCounters * kCntSlots);
+  GR_HIP(hipMemcpy(slots.data(), ctx->d.counters, slots.size() * 8, hipMemcpyDeviceToHost));
   unsigned long long c[kNumCounters];
-  GR_HIP(hipMemcpy(c, ctx->d.counters, sizeof(c), hipMemcpyDeviceToHost));
+  for (int k = 0; k < kNumCounters; ++k) {
+    c[k] = 0;
+    for (int j = 0; j < kCntSlots; ++j) {
+      const unsigned long long v = slots[(size_t)k * kCntSlots + j];
+      c[k] = k == kCntMaxCand ? std::max(c[k], v) : c[k] + v;
+    }
+  }
   grains_stats_t& st = ctx->stats;
   st.steps = steps;
   st.candidate_pairs = c[kCntCandidates];
@@ -878,7 +901,7 @@ static int run_steps(grains_ctx* ctx, uint32_t n_steps, bool graph) {
   int rc = finalize(ctx);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
-  launch_fill_u32((uint32_t*)ctx->d.counters, 2 * kNumCounters, 0u, s);
+  launch_fill_u32((uint32_t*)ctx->d.counters, 2 * kNumCounters * kCntSlots, 0u, s);
   if (graph && !ctx->graph_exec) {
     ctx->capturing = true;
     GR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -939,6 +962,7 @@ extern "C" int grains_set_launch(grains_ctx_t* ctx, int block_size, int narrow_b
   ctx->cfg.narrow_blocks_per_cu = narrow_blocks_per_cu;
   ctx->d.block = (uint32_t)block_size;
   ctx->d.narrow_blocks = (uint32_t)(cu_count() * narrow_blocks_per_cu);
+  if (ctx->d.ecap) set_defer_cap(ctx);
   drop_graph(ctx);  // a captured step holds the old launch shape
   return GRAINS_OK;
 }

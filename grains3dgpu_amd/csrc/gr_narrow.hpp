@@ -109,19 +109,40 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
 // long searches are kept together and the screen's waves stay short.  The
 // queued searches start from scratch with the same arithmetic: results equal
 // those of PASS 0 bit for bit.
+// The screen queues into one segment per block (an LDS counter, no global
+// atomic: a single queue counter would serialise one atomic per wave); the
+// segments are flattened by an exclusive scan of their counts, and PASS 1
+// finds an entry's segment by binary search over the scan.
 template <class T, bool SV, int PASS>
 __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   constexpr bool screen = SV && PASS == 2;
+  __shared__ uint32_t seg_n;
+  uint32_t* seg = d.defer_list + (uint64_t)blockIdx.x * d.defer_cap;
+  if constexpr (screen) {
+    if (threadIdx.x == 0) seg_n = 0;
+    __syncthreads();
+  }
   PairRes<T>* res = (PairRes<T>*)d.pair_res;
-  const uint32_t E = PASS == 1 ? *d.defer_count : d.entry_off[d.n];
+  const uint32_t E = PASS == 1 ? d.defer_off[gridDim.x] : d.entry_off[d.n];
   uint32_t c_gjk = 0, c_it = 0, c_deep = 0, c_ref = 0, c_bk = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride) {
-    const uint32_t e = PASS == 1 ? d.defer_list[k] : (uint32_t)k;
+    uint32_t e = (uint32_t)k;
+    if constexpr (PASS == 1) {
+      uint32_t lo = 0, hi = gridDim.x;  // last segment b with defer_off[b] <= k
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (d.defer_off[mid] <= (uint32_t)k)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      e = d.defer_list[(uint64_t)lo * d.defer_cap + ((uint32_t)k - d.defer_off[lo])];
+    }
     NpStats st = {0, 0, 0, 0, 0};
     PairRes<T> r;
     uint32_t flag = pair_force<T, T, SV, screen>(d, d.pair_i[e], d.pair_j[e], r, st);
-    if constexpr (screen) wave_append(d.defer_list, d.defer_count, st.deferred != 0, e);
+    if constexpr (screen) block_append(seg, &seg_n, st.deferred != 0, e);
     if (!st.deferred) {
       if (flag & kFlagContact) res[e] = r;
       // FP32-mixed: re-run in FP64 when the search ended on rounding
@@ -139,11 +160,15 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
     if constexpr (sizeof(T) == 4 && !screen)
       wave_append(d.refine_list, d.refine_count, !st.deferred && (st.backup || st.deep), e);
   }
-  count_add(&d.counters[kCntGjk], c_gjk);
-  count_add(&d.counters[kCntGjkIters], c_it);
-  count_add(&d.counters[kCntDeep], c_deep);
-  count_add(&d.counters[kCntRefine], c_ref);
-  count_add(&d.counters[kCntBackup], c_bk);
+  if constexpr (screen) {
+    __syncthreads();
+    if (threadIdx.x == 0) d.defer_count[blockIdx.x] = seg_n;
+  }
+  count_add(d.counters, kCntGjk, c_gjk);
+  count_add(d.counters, kCntGjkIters, c_it);
+  count_add(d.counters, kCntDeep, c_deep);
+  count_add(d.counters, kCntRefine, c_ref);
+  count_add(d.counters, kCntBackup, c_bk);
 }
 
 // FP64 re-run of the entries queued by k_narrow<float>; results stored in FP32.
@@ -251,9 +276,9 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
     d.quat[2 * n + i] = qn.z;
     d.quat[3 * n + i] = qn.w;
   }
-  count_add(&d.counters[kCntWall], walls);
+  count_add(d.counters, kCntWall, walls);
   // contacts are counted after any FP64 re-run has settled each entry
-  count_add(&d.counters[kCntContacts], contacts);
+  count_add(d.counters, kCntContacts, contacts);
 }
 
 // GrainsTestDev workload: GJK distance of independent pairs of full shapes.
