@@ -6,8 +6,8 @@
 //                 histogram, device-wide exclusive scan, stable scatter ranked
 //                 with wave ballots (replaces thrust::sort_by_key,
 //                 ComponentManagerGPU.cpp:282-285)
-//   k_cells       cell start/end by adjacent difference
-//                 (ComponentManagerGPU_Kernels.cpp:28-75), the inverse
+//   k_cells       CSR cell offsets from the sorted keys (the cell start/end of
+//                 ComponentManagerGPU_Kernels.cpp:28-75), the inverse
 //                 permutation and the pose records in cell order
 //   k_candidates  per particle, the 27 cells in the reference order
 //                 (LinkedCell.cpp:41-70) and their residents ascending
@@ -241,17 +241,22 @@ void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx
 }
 
 // ---- cell ranges and sorted records -----------------------------------------------
-// One thread per sorted position s: cell start/end by adjacent difference,
-// the inverse permutation, and the particle's records in cell order -- the
-// only gather of the state per step; the candidate and narrow phases read
-// bp / pk of neighbouring cells, which are neighbouring lines.
+// One thread per sorted position s: the CSR offsets of the cells (cell_off[c]
+// = first sorted position whose key is >= c; thread s writes the cells in
+// (key[s-1], key[s]], the last thread also those after its key, so every one
+// of the C + 1 offsets is written once and empty cells get empty ranges), the
+// inverse permutation, and the particle's records in cell order -- the only
+// gather of the state per step; the candidate and narrow phases read bp / pk
+// of neighbouring cells, which are neighbouring lines.
 __global__ void k_cells(DevScene d, const uint32_t* __restrict__ ks, const uint32_t* __restrict__ is) {
   const uint64_t n = d.n;
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t k = ks[s];
-  if (s == 0 || ks[s - 1] != k) d.cell_start[k] = (uint32_t)s;
-  if (s == n - 1 || ks[s + 1] != k) d.cell_end[k] = (uint32_t)s + 1;
+  const uint32_t c0 = s == 0 ? 0u : ks[s - 1] + 1u;
+  for (uint32_t c = c0; c <= k; ++c) d.cell_off[c] = (uint32_t)s;
+  if (s == n - 1)
+    for (uint32_t c = k + 1u; c <= d.ncells; ++c) d.cell_off[c] = (uint32_t)n;
   const uint32_t i = is[s];
   d.rank[i] = (uint32_t)s;
   const uint32_t shp = d.shape[i];
@@ -270,9 +275,10 @@ __global__ void k_cells(DevScene d, const uint32_t* __restrict__ ks, const uint3
 }
 
 void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
-  // empty cells: start = end = 0
-  launch_fill_u32(d.cell_start, d.ncells, 0u, s);
-  launch_fill_u32(d.cell_end, d.ncells, 0u, s);
+  if (d.n == 0) {
+    launch_fill_u32(d.cell_off, (uint64_t)d.ncells + 1, 0u, s);
+    return;
+  }
   hipLaunchKernelGGL(k_cells, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d, keys_s,
                      idx_s);
 }
@@ -283,7 +289,27 @@ void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx
 // cand[k * N + s] (k-major, so the lanes of a wave write neighbouring words)
 // and their count into cand_count[s].  Residents of a cell ascend in sorted
 // position exactly as in particle index (the sort is stable on an identity
-// payload).
+// payload), and the three cells x-1, x, x+1 of a neighbour row have
+// consecutive keys, so away from the x borders a row is one range
+// [off[c-1], off[c+2]) visited in the reference order.
+__device__ __forceinline__ void cand_scan(const DevScene& d, uint32_t t0, uint32_t t1, uint32_t s, V3<double> xi,
+                                          double ri, const SimGrid& g, uint32_t& cnt, uint32_t& over) {
+  const uint64_t n = d.n;
+  for (uint32_t t = t0; t < t1; ++t) {
+    if (t == s) continue;
+    const double4 bj = d.bp[t];
+    V3<double> dd = V3<double>{bj.x, bj.y, bj.z} - xi;
+    if (g.periodic) dd = min_image(g, dd);
+    const double rs = ri + bj.w;
+    if (!(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z < rs * rs * (1.0 + kPrefilterSlack))) continue;
+    if (cnt < d.K)
+      d.cand[(uint64_t)cnt * n + s] = t;
+    else
+      over = 1;
+    ++cnt;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_t* __restrict__ ks) {
   const uint64_t n = d.n;
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -293,42 +319,36 @@ __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_
     const SimGrid g = d.grids[d.pk[s].sim];
     const V3<double> xi = {bi.x, bi.y, bi.z};
     const double ri = bi.w;
+    const int nx = (int)g.n3[0], ny = (int)g.n3[1], nz = (int)g.n3[2];
     const uint32_t local = ks[s] - g.base;
-    const int cx = (int)(local % g.n3[0]);
-    const int cy = (int)((local / g.n3[0]) % g.n3[1]);
-    const int cz = (int)(local / (g.n3[0] * g.n3[1]));
-    const uint32_t K = d.K;
+    const int cx = (int)(local % (uint32_t)nx);
+    const int cy = (int)((local / (uint32_t)nx) % (uint32_t)ny);
+    const int cz = (int)(local / ((uint32_t)nx * (uint32_t)ny));
+    const bool px = g.periodic & 1, py = g.periodic & 2, pz = g.periodic & 4;
+    const bool row_ok = cx >= 1 && cx + 1 < nx;  // x-1 .. x+1 inside, no wrap
     for (int dz = -1; dz <= 1; ++dz) {
       int z = cz + dz;
-      if (g.periodic & 4) z = (z + (int)g.n3[2]) % (int)g.n3[2];
-      if (z < 0 || z >= (int)g.n3[2]) continue;
+      if (pz) z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+      if (z < 0 || z >= nz) continue;
       for (int dy = -1; dy <= 1; ++dy) {
         int y = cy + dy;
-        if (g.periodic & 2) y = (y + (int)g.n3[1]) % (int)g.n3[1];
-        if (y < 0 || y >= (int)g.n3[1]) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          int x = cx + dx;
-          if (g.periodic & 1) x = (x + (int)g.n3[0]) % (int)g.n3[0];
-          if (x < 0 || x >= (int)g.n3[0]) continue;
-          const uint32_t cell = g.base + ((uint32_t)z * g.n3[1] + (uint32_t)y) * g.n3[0] + (uint32_t)x;
-          const uint32_t t1 = d.cell_end[cell];
-          for (uint32_t t = d.cell_start[cell]; t < t1; ++t) {
-            if (t == (uint32_t)s) continue;
-            const double4 bj = d.bp[t];
-            V3<double> dd = V3<double>{bj.x, bj.y, bj.z} - xi;
-            if (g.periodic) dd = min_image(g, dd);
-            const double rs = ri + bj.w;
-            if (!(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z < rs * rs * (1.0 + kPrefilterSlack))) continue;
-            if (cnt < K)
-              d.cand[(uint64_t)cnt * n + s] = t;
-            else
-              over = 1;
-            ++cnt;
+        if (py) y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        if (y < 0 || y >= ny) continue;
+        const uint32_t row = g.base + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+        if (row_ok) {
+          cand_scan(d, d.cell_off[row + cx - 1], d.cell_off[row + cx + 2], (uint32_t)s, xi, ri, g, cnt, over);
+        } else {
+          for (int dx = -1; dx <= 1; ++dx) {
+            int x = cx + dx;
+            if (px) x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+            if (x < 0 || x >= nx) continue;
+            const uint32_t c = row + (uint32_t)x;
+            cand_scan(d, d.cell_off[c], d.cell_off[c + 1], (uint32_t)s, xi, ri, g, cnt, over);
           }
         }
       }
     }
-    d.cand_count[s] = cnt < K ? cnt : K;
+    d.cand_count[s] = cnt < d.K ? cnt : d.K;
   }
   count_add(d.counters, kCntOverflow, over);
   count_max(d.counters, kCntMaxCand, cnt);

@@ -662,7 +662,7 @@ static int finalize(grains_ctx* ctx) {
       (rc = dalloc(ctx, &potr, (otr.size() + 12) * sizeof(double), L)) ||
       (rc = dalloc(ctx, &d.key, n * 4, L)) || (rc = dalloc(ctx, &d.keyA, n * 4, L)) || (rc = dalloc(ctx, &d.idxA, n * 4, L)) ||
       (rc = dalloc(ctx, &d.keyB, n * 4, L)) || (rc = dalloc(ctx, &d.idxB, n * 4, L)) || (rc = dalloc(ctx, &d.rank, n * 4, L)) ||
-      (rc = dalloc(ctx, &d.cell_start, (size_t)d.ncells * 4, L)) || (rc = dalloc(ctx, &d.cell_end, (size_t)d.ncells * 4, L)) ||
+      (rc = dalloc(ctx, &d.cell_off, ((size_t)d.ncells + 1) * 4, L)) ||
       (rc = dalloc(ctx, &d.bp, n * sizeof(double4), L)) || (rc = dalloc(ctx, &d.pk, n * sizeof(PoseRec), L)) ||
       (rc = dalloc(ctx, &d.hist, (256 * ntiles + 1) * 4, L)) || (rc = dalloc(ctx, &d.scan_blocks, nblk * 4, L)) ||
       (rc = dalloc(ctx, &d.cand, ecap * 4, L)) || (rc = dalloc(ctx, &d.cand_count, n * 4, L)) ||

@@ -6,7 +6,7 @@
 //   state    pos[3][N], quat[4][N] (x,y,z,w), vel[3][N], omega[3][N]   FP64 planes
 //   ids      shape[N], sim[N]                                          u32
 //   broad    key[N] (by particle), key/idx ping-pong [2][N], rank[N],
-//            cell_start[C], cell_end[C]                                u32
+//            cell_off[C+1] (first sorted position of each cell)      u32
 //   sorted   bp[N] = {x, y, z, prefilter radius}   32 B, cell order
 //            pk[N] = PoseRec {pos, quat, shape, sim}  64 B, cell order
 //            (rebuilt every step from the planes; neighbours of a cell are
@@ -111,7 +111,7 @@ struct DevScene {
   // broad phase
   uint32_t *key, *keyA, *idxA, *keyB, *idxB, *rank;
   const uint32_t* sidx;         // particle at each sorted position (this step)
-  uint32_t *cell_start, *cell_end;
+  uint32_t* cell_off;           // CSR over cells: residents of c are [off[c], off[c+1])
   double4* bp;                  // {x, y, z, prefilter radius} by sorted position
   PoseRec* pk;                  // pose by sorted position
   uint32_t ncells;
