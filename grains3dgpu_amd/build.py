@@ -21,6 +21,7 @@ UNITS = {
     "gr_narrow64": "off",
     "gr_narrow32": "fast",
     "gr_capi": "off",
+    "gr_adjoint": "off",
 }
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
           "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]

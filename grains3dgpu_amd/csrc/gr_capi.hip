@@ -319,6 +319,15 @@ struct grains_ctx {
   std::vector<hipEvent_t> pev;
   uint32_t pev_used = 0;
   double phase_ms[GRAINS_NUM_PHASES] = {0};
+  // differentiation: the recorded trajectory and the sweep's buffers
+  bool adj_rec = false;
+  uint32_t adj_every = 0;
+  uint64_t adj_steps = 0;
+  std::vector<void*> adj_cp;      // state every adj_every steps (13 planes)
+  std::vector<void*> adj_allocs;  // sweep buffers
+  AdjBufs ab{};
+  double *adj_blk = nullptr, *adj_save = nullptr;
+  int32_t* adj_count = nullptr;
 };
 
 #define GR_HIP(call)                                                                    \
@@ -358,6 +367,8 @@ static void free_events(grains_ctx* ctx) {
   for (hipEvent_t e : ctx->pev) (void)hipEventDestroy(e);
   ctx->pev.clear();
 }
+
+static void adj_drop(grains_ctx* ctx);
 
 static void drop_graph(grains_ctx* ctx) {
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -410,6 +421,7 @@ extern "C" void grains_destroy(grains_ctx_t* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
   free_events(ctx);
+  adj_drop(ctx);
   free_list(ctx->allocs);
   free_list(ctx->state_allocs);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -529,11 +541,22 @@ static void set_defer_cap(grains_ctx* ctx) {
   d.defer_cap = (uint32_t)(((d.ecap + per - 1) / per) * d.block);
 }
 
+static void adj_drop(grains_ctx* ctx) {
+  free_list(ctx->adj_cp);
+  free_list(ctx->adj_allocs);
+  ctx->adj_rec = false;
+  ctx->adj_steps = 0;
+  ctx->ab = AdjBufs{};
+  ctx->adj_blk = ctx->adj_save = nullptr;
+  ctx->adj_count = nullptr;
+}
+
 static int finalize(grains_ctx* ctx) {
   if (ctx->layout_ok) return GRAINS_OK;
   if (!ctx->n) return fail(ctx, GRAINS_ESTATE, "no particles");
   if (ctx->tables.empty()) return fail(ctx, GRAINS_ESTATE, "no contact table");
   drop_graph(ctx);
+  adj_drop(ctx);  // a new layout ends a recorded trajectory
   free_list(ctx->allocs);
   DevScene& d = ctx->d;
   const bool quirks = (ctx->cfg.flags & GRAINS_REF_QUIRKS) != 0;
@@ -897,6 +920,36 @@ static int read_stats(grains_ctx* ctx, uint32_t steps) {
   return GRAINS_OK;
 }
 
+// ---- differentiation: recorded trajectory --------------------------------------
+// the live state <-> 13 planes (pos 3, quat 4, vel 3, omega 3), on the stream
+static int state_to(grains_ctx* ctx, double* dst) {
+  const uint64_t n = ctx->n;
+  hipStream_t s = ctx->stream;
+  GR_HIP(hipMemcpyAsync(dst, ctx->d.pos, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+  GR_HIP(hipMemcpyAsync(dst + 3 * n, ctx->d.quat, 4 * n * 8, hipMemcpyDeviceToDevice, s));
+  GR_HIP(hipMemcpyAsync(dst + 7 * n, ctx->d.vel, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+  GR_HIP(hipMemcpyAsync(dst + 10 * n, ctx->d.omega, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+  return GRAINS_OK;
+}
+static int state_from(grains_ctx* ctx, const double* src) {
+  const uint64_t n = ctx->n;
+  hipStream_t s = ctx->stream;
+  GR_HIP(hipMemcpyAsync(ctx->d.pos, src, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+  GR_HIP(hipMemcpyAsync(ctx->d.quat, src + 3 * n, 4 * n * 8, hipMemcpyDeviceToDevice, s));
+  GR_HIP(hipMemcpyAsync(ctx->d.vel, src + 7 * n, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+  GR_HIP(hipMemcpyAsync(ctx->d.omega, src + 10 * n, 3 * n * 8, hipMemcpyDeviceToDevice, s));
+  return GRAINS_OK;
+}
+// after each recorded step: a checkpoint every adj_every steps
+static int adj_after_step(grains_ctx* ctx) {
+  ++ctx->adj_steps;
+  if (ctx->adj_steps % ctx->adj_every) return GRAINS_OK;
+  double* cp;
+  int rc = dalloc(ctx, &cp, 13 * ctx->n * 8, ctx->adj_cp);
+  if (rc) return rc;
+  return state_to(ctx, cp);
+}
+
 static int run_steps(grains_ctx* ctx, uint32_t n_steps, bool graph) {
   int rc = finalize(ctx);
   if (rc) return rc;
@@ -918,6 +971,7 @@ static int run_steps(grains_ctx* ctx, uint32_t n_steps, bool graph) {
       GR_HIP(hipGraphLaunch(ctx->graph_exec, s));
     else
       enqueue_step(ctx);
+    if (ctx->adj_rec && (rc = adj_after_step(ctx))) return rc;
   }
   GR_HIP(hipGetLastError());
   GR_HIP(hipEventRecord(ctx->ev1, s));
@@ -1232,5 +1286,119 @@ extern "C" int grains_restore(grains_ctx_t* ctx, const void* bytes, size_t size)
   GR_HIP(hipMemcpy(ctx->d.omega, p, 3 * n * 8, hipMemcpyHostToDevice));
   p += 3 * n * 8;
   GR_HIP(hipMemcpy(ctx->d.contact_count, p, n * 4, hipMemcpyHostToDevice));
+  return GRAINS_OK;
+}
+
+// ---- differentiation: the reverse sweep ---------------------------------------------
+extern "C" int grains_adjoint_record(grains_ctx_t* ctx, uint32_t checkpoint_every) {
+  if (!ctx || checkpoint_every == 0) return GRAINS_EINVAL;
+  int rc = finalize(ctx);
+  if (rc) return rc;
+  if (ctx->cfg.flags & GRAINS_REF_QUIRKS)
+    return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers the physics semantics (no REF_QUIRKS)");
+  for (uint32_t s : ctx->pshape) {
+    const grains_shape_t& h = ctx->shapes[s].s;
+    const bool ell = h.kind == GRAINS_SUPERQUADRIC && h.dims[3] == 2.0 && h.dims[4] == 2.0;
+    if (h.kind != GRAINS_SPHERE && !ell)
+      return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers spheres and ellipsoids (superquadrics n1 = n2 = 2)");
+  }
+  for (const auto& t : ctx->tables)
+    if (t.kr != 0.0) return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers kr = 0 (no rolling resistance)");
+  // the sweep's buffers are sized by checkpoint_every: drop them
+  free_list(ctx->adj_cp);
+  free_list(ctx->adj_allocs);
+  ctx->ab = AdjBufs{};
+  ctx->adj_blk = ctx->adj_save = nullptr;
+  ctx->adj_count = nullptr;
+  ctx->adj_every = checkpoint_every;
+  ctx->adj_steps = 0;
+  ctx->adj_rec = true;
+  double* cp;
+  if ((rc = dalloc(ctx, &cp, 13 * ctx->n * 8, ctx->adj_cp)) || (rc = state_to(ctx, cp))) return rc;
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  return GRAINS_OK;
+}
+
+extern "C" int grains_adjoint(grains_ctx_t* ctx, const double* dL_dstate, double* dL_dparams, double* dL_dstate0) {
+  if (!ctx || !dL_dstate || !dL_dparams) return GRAINS_EINVAL;
+  if (!ctx->adj_rec) return fail(ctx, GRAINS_ESTATE, "no recorded trajectory (grains_adjoint_record)");
+  const uint64_t n = ctx->n, ecap = ctx->d.ecap, T = ctx->adj_steps, K = ctx->adj_every;
+  const uint32_t ntab = (uint32_t)ctx->tables.size();
+  hipStream_t s = ctx->stream;
+  int rc;
+  AdjBufs& a = ctx->ab;
+  if (!ctx->adj_blk) {
+    uint32_t max_obs = 1;
+    for (const SimGrid& g : ctx->grids) max_obs = std::max(max_obs, g.obs_end - g.obs_begin);
+    auto& L = ctx->adj_allocs;
+    if ((rc = dalloc(ctx, &ctx->adj_blk, K * 13 * n * 8, L)) || (rc = dalloc(ctx, &ctx->adj_save, 13 * n * 8, L)) ||
+        (rc = dalloc(ctx, &ctx->adj_count, n * 4, L)) || (rc = dalloc(ctx, &a.abar, 13 * n * 8, L)) ||
+        (rc = dalloc(ctx, &a.own, 13 * n * 8, L)) || (rc = dalloc(ctx, &a.Fb, 3 * n * 8, L)) ||
+        (rc = dalloc(ctx, &a.Mb, 3 * n * 8, L)) || (rc = dalloc(ctx, &a.part_i, ecap * 13 * 8, L)) ||
+        (rc = dalloc(ctx, &a.part_j, ecap * 13 * 8, L)) || (rc = dalloc(ctx, &a.entry_theta, ecap * 4 * 8, L)) ||
+        (rc = dalloc(ctx, &a.entry_tidx, ecap * 4, L)) || (rc = dalloc(ctx, &a.wall_theta, n * max_obs * 4 * 8, L)) ||
+        (rc = dalloc(ctx, &a.wall_tidx, n * max_obs * 4, L)) || (rc = dalloc(ctx, &a.dtheta, (size_t)ntab * 4 * 8, L)))
+      return rc;
+    a.max_obs = max_obs;
+  }
+  // the state adjoint, rows -> planes
+  {
+    std::vector<double> planes(13 * n);
+    const double* pos = dL_dstate;
+    const double* quat = pos + 3 * n;
+    const double* vel = quat + 4 * n;
+    const double* om = vel + 3 * n;
+    for (uint64_t i = 0; i < n; ++i) {
+      for (int k = 0; k < 3; ++k) planes[k * n + i] = pos[3 * i + k];
+      for (int k = 0; k < 4; ++k) planes[(3 + k) * n + i] = quat[4 * i + k];
+      for (int k = 0; k < 3; ++k) planes[(7 + k) * n + i] = vel[3 * i + k];
+      for (int k = 0; k < 3; ++k) planes[(10 + k) * n + i] = om[3 * i + k];
+    }
+    GR_HIP(hipMemcpy(a.abar, planes.data(), 13 * n * 8, hipMemcpyHostToDevice));
+  }
+  launch_fill_u32((uint32_t*)a.dtheta, (uint64_t)ntab * 8, 0u, s);
+  if ((rc = state_to(ctx, ctx->adj_save))) return rc;
+  GR_HIP(hipMemcpyAsync(ctx->adj_count, ctx->d.contact_count, n * 4, hipMemcpyDeviceToDevice, s));
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  const uint64_t nblocks = (T + K - 1) / K;
+  for (uint64_t bb = nblocks; bb-- > 0;) {
+    const uint64_t t0 = bb * K, t1 = std::min(t0 + K, T);
+    if ((rc = state_from(ctx, (const double*)ctx->adj_cp[bb]))) break;
+    for (uint64_t t = t0; t < t1 && !rc; ++t) {
+      if ((rc = state_to(ctx, ctx->adj_blk + (t - t0) * 13 * n))) break;
+      enqueue_step(ctx);
+    }
+    for (uint64_t t = t1; t-- > t0 && !rc;) {
+      double* st = ctx->adj_blk + (t - t0) * 13 * n;
+      if ((rc = state_from(ctx, st))) break;
+      enqueue_step(ctx);  // entries, contact flags, force and torque of step t
+      a.state = st;
+      launch_adjoint_step(ctx->d, a, ntab, s);
+    }
+    if (rc) break;
+  }
+  ctx->profiling = prof;
+  int rc2 = state_from(ctx, ctx->adj_save);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  GR_HIP(hipMemcpyAsync(ctx->d.contact_count, ctx->adj_count, n * 4, hipMemcpyDeviceToDevice, s));
+  GR_HIP(hipGetLastError());
+  GR_HIP(hipStreamSynchronize(s));
+  GR_HIP(hipMemcpy(dL_dparams, a.dtheta, (size_t)ntab * 4 * 8, hipMemcpyDeviceToHost));
+  if (dL_dstate0) {
+    std::vector<double> planes(13 * n);
+    GR_HIP(hipMemcpy(planes.data(), a.abar, 13 * n * 8, hipMemcpyDeviceToHost));
+    double* pos = dL_dstate0;
+    double* quat = pos + 3 * n;
+    double* vel = quat + 4 * n;
+    double* om = vel + 3 * n;
+    for (uint64_t i = 0; i < n; ++i) {
+      for (int k = 0; k < 3; ++k) pos[3 * i + k] = planes[k * n + i];
+      for (int k = 0; k < 4; ++k) quat[4 * i + k] = planes[(3 + k) * n + i];
+      for (int k = 0; k < 3; ++k) vel[3 * i + k] = planes[(7 + k) * n + i];
+      for (int k = 0; k < 3; ++k) om[3 * i + k] = planes[(10 + k) * n + i];
+    }
+  }
   return GRAINS_OK;
 }

@@ -145,6 +145,21 @@ struct DevScene {
   int precision;
 };
 
+// Buffers of one backward step of the adjoint sweep (gr_adjoint.hip).
+struct AdjBufs {
+  double* state;          // 13 planes (pos 3, quat 4, vel 3, omega 3): the state at step t
+  double* abar;           // 13 planes: in, the adjoint of the state at t+1; out, at t
+  double* own;            // 13 planes: a particle's own partial (integration, walls)
+  double *Fb, *Mb;        // 3 planes each: adjoint of the total force and torque
+  double *part_i, *part_j;  // 13 per entry: the entry's partials for its two particles
+  double* entry_theta;    // 4 per entry: kn, en, etat, muc
+  uint32_t* entry_tidx;   // contact-table entry of each entry (~0: none)
+  double* wall_theta;     // 4 per (particle, wall slot)
+  uint32_t* wall_tidx;
+  uint32_t max_obs;       // wall slots per particle
+  double* dtheta;         // 4 per contact-table entry, summed over the sweep
+};
+
 // ---- launchers ------------------------------------------------------------------
 // gr_broad.hip
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
@@ -165,6 +180,8 @@ void launch_narrow32(const DevScene& d, int pass, hipStream_t s);
 void launch_refine64(const DevScene& d, hipStream_t s);
 void launch_integrate64(const DevScene& d, hipStream_t s);
 void launch_integrate32(const DevScene& d, hipStream_t s);
+// gr_adjoint.hip: one backward step (the forward step t must have just run)
+void launch_adjoint_step(const DevScene& d, const AdjBufs& a, uint32_t n_tables, hipStream_t s);
 // standalone GJK distance batches (GrainsTestDev workload)
 void launch_gjk_batch64(const ShapeNP<double>* shapes, const double* verts, const uint32_t* sa,
                         const uint32_t* sb, const double* ta, const double* tb, uint64_t n,

@@ -53,6 +53,8 @@ def lib():
         L.grains_set_profiling.argtypes = [vp, C.c_int]
         L.grains_set_launch.argtypes = [vp, C.c_int, C.c_int]
         L.grains_set_narrow_passes.argtypes = [vp, C.c_int]
+        L.grains_adjoint_record.argtypes = [vp, C.c_uint32]
+        L.grains_adjoint.argtypes = [vp, vp, vp, vp]
         L.grains_get_phase_times.argtypes = [vp, vp, C.c_int]
         L.grains_snapshot.argtypes = [vp, vp, C.POINTER(C.c_size_t)]
         L.grains_restore.argtypes = [vp, vp, C.c_size_t]
@@ -108,7 +110,34 @@ class Engine:
 
     def set_contact_tables(self, sc):
         arr = sc.table_array()
+        self._n_table_entries = len(arr)
         self._chk(self.L.grains_set_contact_tables(self.h, C.cast(arr, vp), sc.n_materials, sc.n_tables))
+
+    # ---- differentiation ------------------------------------------------------
+    def adjoint_record(self, checkpoint_every=100):
+        """Start a recorded trajectory at the current state (checkpoint every
+        `checkpoint_every` steps of the following step calls)."""
+        self._chk(self.L.grains_adjoint_record(self.h, checkpoint_every))
+
+    def adjoint(self, d_pos, d_quat, d_vel, d_omega, want_state0=False):
+        """Reverse sweep over the recorded steps.  Inputs: the loss gradient
+        at the current state (n x 3, n x 4, n x 3, n x 3).  Returns the
+        gradient with respect to the contact-table entries, an array
+        (entries, 4) of (kn, en, etat, muc), and with want_state0 the
+        gradients with respect to the recorded initial state."""
+        n = self.n
+        ds = np.ascontiguousarray(np.concatenate([np.asarray(d_pos, np.float64).reshape(-1),
+                                                  np.asarray(d_quat, np.float64).reshape(-1),
+                                                  np.asarray(d_vel, np.float64).reshape(-1),
+                                                  np.asarray(d_omega, np.float64).reshape(-1)]))
+        assert ds.size == 13 * n
+        dp = np.zeros((self._n_table_entries, 4))
+        s0 = np.zeros(13 * n) if want_state0 else None
+        self._chk(self.L.grains_adjoint(self.h, A.dptr(ds), A.dptr(dp), A.dptr(s0) if s0 is not None else None))
+        if not want_state0:
+            return dp
+        return dp, (s0[:3 * n].reshape(n, 3), s0[3 * n:7 * n].reshape(n, 4), s0[7 * n:10 * n].reshape(n, 3),
+                    s0[10 * n:].reshape(n, 3))
 
     def set_sims(self, sc):
         arr = sc.sim_array()

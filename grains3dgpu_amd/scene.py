@@ -453,3 +453,79 @@ def config3(n_sims=512, n_range=(500, 20_000), seed=3, phi=0.3, jitter=0.05, cru
     sc.notes = dict(sizes=sizes.tolist(), kn=kn.tolist(), mu=mu.tolist(), rounds=rounds, redraws=redraws,
                     steps=1000)
     return sc
+
+
+def config4(n=8192, phi=0.35, seed=4, kn=1e5, en=0.8, mu=0.3, vel=0.1, crust_frac=0.05, max_rounds=60,
+            precision=None, log=None, resolve=True):
+    """Differentiable calibration: n particles, half spheres (material 0, r ~
+    U[0.4, 0.6] mm), half ellipsoids (material 1, superquadrics n1 = n2 = 2,
+    a ~ U[0.4, 0.6] mm, b and c = a U[0.5, 1]: axis ratios up to 2:1), rho =
+    2500, on a jittered cubic lattice at solid fraction phi in a walled box,
+    random orientations, Gaussian velocities (sigma `vel` m/s) so that
+    collisions start at once; gravity -9.81 z, dt = 1e-6.  Contact tables:
+    one block of the three material pairs (0-0, 0-1, 1-1; walls are material
+    0) with the same kn, en, mu and the tangential damping matched to the
+    normal one at these values (kept fixed: the calibrated parameters are kn,
+    en and mu per pair).  Overlaps of the lattice are removed with the
+    library's own narrow phase: the higher index of each overlapping pair
+    gets a new orientation, from round 4 its size shrunk by 0.9."""
+    rng = np.random.default_rng(seed)
+    kind = np.where(np.arange(n) % 2 == 0, A.SPHERE, A.SUPERQUADRIC).astype(np.int32)
+    a = rng.uniform(0.4e-3, 0.6e-3, n)
+    dims = np.zeros((n, 5))
+    dims[:, 0] = a
+    sq = kind == A.SUPERQUADRIC
+    dims[sq, 1] = a[sq] * rng.uniform(0.5, 1.0, sq.sum())
+    dims[sq, 2] = a[sq] * rng.uniform(0.5, 1.0, sq.sum())
+    dims[sq, 3:] = 2.0
+    vol = np.where(sq, 4.0 / 3.0 * np.pi * dims[:, 0] * dims[:, 1] * dims[:, 2], 4.0 / 3.0 * np.pi * a ** 3)
+    sp = (float(vol.mean()) / phi) ** (1.0 / 3.0)
+    side = int(math.ceil(n ** (1.0 / 3.0)))
+    idx = np.arange(n)
+    cz, rem = np.divmod(idx, side * side)
+    cy, cx = np.divmod(rem, side)
+    margin = 0.7e-3
+    L = side * sp + 2 * margin
+    pos = margin + (np.stack([cx, cy, cz], 1) + 0.5) * sp + rng.uniform(-0.05, 0.05, (n, 3)) * sp
+    quat = random_quaternions(rng, n)
+    rho = 2500.0
+    m0 = rho * float(vol.mean())
+    etat = matched_etat(kn, en, 0.5 * m0)
+    tables = np.array([hooke_row(kn, en, etat, mu)] * 3)
+
+    def make():
+        shapes = [dict(kind=int(kind[i]), material=int(kind[i] == A.SUPERQUADRIC),
+                       dims=tuple(dims[i, :1] if kind[i] == A.SPHERE else dims[i]),
+                       crust=crust_frac * float(dims[i, :1 if kind[i] == A.SPHERE else 3].min()), density=rho)
+                  for i in range(n)]
+        ws, otr = box_walls([0, 0, 0], [L, L, L])
+        return Scene(shapes=shapes + ws, tables=tables, n_materials=2,
+                     sims=[dict(domain_min=(0, 0, 0), domain_max=(L, L, L), gravity=(0, 0, -9.81))],
+                     particle_shape=np.arange(n, dtype=np.uint32), pos=pos.copy(), quat=quat.copy(),
+                     vel=np.random.default_rng(seed + 1).normal(0.0, vel, (n, 3)), omega=np.zeros((n, 3)),
+                     obstacle_shape=np.arange(n, n + len(ws), dtype=np.uint32), obstacle_tr=otr, dt=1e-6,
+                     name=f"c4_calibration_{n}")
+    rounds = 0
+    if resolve:
+        from .engine import Engine
+        prec = A.FP64 if precision is None else precision
+        for rounds in range(1, max_rounds + 1):
+            sc = make()
+            eng = load(Engine(dt=sc.dt, precision=prec, max_neighbors=64), sc)
+            eng.step(1)
+            pairs = eng.get_pairs(A.PAIRS_CONTACT)
+            eng.close()
+            bad = np.unique(np.maximum(pairs[:, 0], pairs[:, 1])) if len(pairs) else np.zeros(0, int)
+            if log:
+                log(f"config4 round {rounds}: {len(pairs) // 2} overlapping pairs")
+            if len(bad) == 0:
+                break
+            quat[bad] = random_quaternions(rng, len(bad))
+            if rounds >= 4:
+                dims[bad, :3] *= 0.9
+        else:
+            raise RuntimeError("config4: overlaps remain")
+    sc = make()
+    vol = np.where(sq, 4.0 / 3.0 * np.pi * dims[:, 0] * dims[:, 1] * dims[:, 2], 4.0 / 3.0 * np.pi * dims[:, 0] ** 3)
+    sc.notes = dict(phi=float(vol.sum() / (L - 2 * margin) ** 3), rounds=rounds, steps=2000)
+    return sc

@@ -239,6 +239,26 @@ int grains_get_stats(grains_ctx_t* ctx, grains_stats_t* stats);
 int grains_set_profiling(grains_ctx_t* ctx, int on);
 int grains_get_phase_times(grains_ctx_t* ctx, double* ms, int n_phases);
 
+/* Differentiation ------------------------------------------------------------
+ * No counterpart in the reference.  Reverse-mode gradients of a trajectory
+ * with respect to the contact-table parameters, for the physics semantics,
+ * spheres and ellipsoids (superquadrics n1 = n2 = 2), Rectangle walls and
+ * kr = 0.  The contact set of every step is the one the forward step finds
+ * (held fixed within the step); contact geometry, Hooke force and
+ * integration are differentiated exactly (separated crust cores by implicit
+ * differentiation of the closest-point conditions).
+ * grains_adjoint_record starts a trajectory at the current state and keeps a
+ * checkpoint every `checkpoint_every` steps of the following grains_step /
+ * grains_step_graph calls.  grains_adjoint then sweeps back over every
+ * recorded step, re-running each block of steps from its checkpoint:
+ * dL_dstate (13 n: pos n x 3, quat n x 4, vel n x 3, omega n x 3, the
+ * gradient of the loss at the current state) -> dL_dparams (n_table_entries
+ * x 4: kn, en, etat, muc, in the order of grains_set_contact_tables) and,
+ * when not NULL, dL_dstate0 (13 n, the gradient with respect to the recorded
+ * initial state).  The current state is left as it was. */
+int grains_adjoint_record(grains_ctx_t* ctx, uint32_t checkpoint_every);
+int grains_adjoint(grains_ctx_t* ctx, const double* dL_dstate, double* dL_dparams, double* dL_dstate0);
+
 /* Narrow phase alone ------------------------------------------------------- */
 /* GJK distance between shape_a[k] at transform ta[k] and shape_b[k] at tb[k]
  * (12-double Transform3 buffers), the GrainsTestDev microbenchmark
