@@ -13,6 +13,12 @@ plain FP64 path beside it, the FP32-mixed position error against FP64 after
 the same steps, GJK pair tests/s (in the step and on the reference's GJK
 microbenchmark), the roofline of the dominant kernel measured with HIP events
 over the timed region, and the FP64 CPU oracle timed on this host.
+
+--workload selects another workload, with a line of the same form:
+c2 polydisperse polyhedra (250k cubes, hulls and spheres, 2,000 steps at
+dt = 5e-7), c3 the batched ensemble (512 beds, ~2.7M particles, autotuned
+launch, one graph), c4 the differentiable calibration (8,192 spheres and
+ellipsoids, forward + adjoint gradients against finite differences).
 """
 import argparse
 import json
@@ -70,15 +76,17 @@ def log(*a):
 GJK_TOL = {A.FP64: 1e-10, A.FP32_MIXED: 1e-6}
 
 
-def run_engine(sc, precision, warmup, steps, profile=True, flags=0, gjk_tol=None, graph=True, profile_steps=100):
+def run_engine(sc, precision, warmup, steps, profile=True, flags=0, gjk_tol=None, graph=True, profile_steps=100,
+               max_neighbors=64, **launch):
     """W warm-up steps, then (profile) min(profile_steps, steps) eagerly
     launched steps with HIP events around every phase, then `steps` timed
     steps replayed from a captured graph (the production mode).  Returns the
     timed wall time, the counters of the timed steps and per-step phase
-    times."""
+    times.  `launch`: block_size, narrow_blocks_per_cu, narrow_passes."""
     from grains3dgpu_amd.engine import Engine
     tol = GJK_TOL[precision] if gjk_tol is None else gjk_tol
-    e = S.load(Engine(dt=sc.dt, precision=precision, max_neighbors=64, gjk_rel_tol=tol, flags=flags), sc)
+    e = S.load(Engine(dt=sc.dt, precision=precision, max_neighbors=max_neighbors, gjk_rel_tol=tol, flags=flags,
+                      **launch), sc)
     if warmup:
         e.step(warmup)
     phases = None
@@ -162,9 +170,69 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_bytes_per_dispatch": nbytes, "dispatches": reps}), flush=True)
 
 
-def cpu_baseline(seconds, threads=1):
+def shape_bytes(sc):
+    """Narrow-phase shape records (ShapeNP<float>, 96 B) of the shapes the
+    particles use, plus their polyhedron vertices (FP32, 12 B each)."""
+    used = np.unique(sc.particle_shape)
+    nv = 0
+    if len(sc.vertices):
+        sh = sc.shapes
+        nv = int(sum(int(sh[int(k)]["n_vertices"]) for k in used))
+    return len(used) * B_NP_SHAPE + 12 * nv
+
+
+def narrow_roofline(sc, r, K, traffic_file=None):
+    """Roofline of the screening narrow pass (k_narrow<float,true,2>, one
+    launch per step): compulsory bytes per launch over its HIP-event time."""
+    st = r["stats"]
+    nar_ms = r["phases"]["narrow"]
+    E, C = st["candidate_pairs"] / K, st["contacts"] / K
+    nbytes = sc.n * B_NP_PARTICLE + shape_bytes(sc) + E * B_NP_ENTRY + C * B_NP_CONTACT
+    achieved_gbs = nbytes / (nar_ms * 1e-3) / 1e9
+    # HBM bytes per launch of the same kernel, from the FETCH_SIZE and
+    # WRITE_SIZE counter runs summarised in profiles/ (a line
+    # "hbm_bytes_per_launch = N"); null until measured
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", traffic_file) if traffic_file else None
+    if tf and os.path.exists(tf):
+        for line in open(tf):
+            if line.strip().startswith("hbm_bytes_per_launch"):
+                traffic = float(line.split("=")[1])
+    return dict(bound="hbm", kernel="k_narrow<float,true,2>", achieved=achieved_gbs, peak=PEAK_HBM_GBS,
+                unit="GB/s", frac=achieved_gbs / PEAK_HBM_GBS, traffic=traffic,
+                bytes_per_launch=nbytes, avg_launch_ms=nar_ms,
+                note="screening narrow pass (every candidate entry); compulsory bytes: 64 B pose+ids per "
+                     "particle, 96 B per shape record (+12 B per polyhedron vertex), 12 B per candidate "
+                     "entry, 24 B per contact; the launch is gather-latency bound")
+
+
+def valu_roofline(r, K):
+    st = r["stats"]
+    ops = OPS_PER_GJK_ITER * st["gjk_iterations"] / K + OPS_PER_GJK_PAIR * st["gjk_calls"] / K + \
+        OPS_PER_ENTRY * st["candidate_pairs"] / K
+    nar_all_ms = r["phases"]["narrow"] + r["phases"]["narrow_deferred"]
+    achieved = ops / (nar_all_ms * 1e-3) / 1e12
+    return dict(bound="valu", kernel="k_narrow<float,true,2|1>", achieved=achieved, peak=PEAK_FP32_TFLOPS,
+                unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS, ops_per_launch=ops,
+                note="no MFMA: no dense contraction on the path; peak = FP32 vector")
+
+
+def pos_error(p32, p64, pos0):
+    d = np.abs(p32["pos"] - p64["pos"]).max()
+    disp = np.abs(p64["pos"] - pos0).max()
+    return dict(pos_rel=float(d / np.abs(p64["pos"]).max()), pos_rel_displacement=float(d / max(disp, 1e-300)),
+                vel_rel=float(np.abs(p32["vel"] - p64["vel"]).max() / max(np.abs(p64["vel"]).max(), 1e-300)),
+                max_displacement_m=float(disp))
+
+
+def step_stats(st):
+    return {k: st[k] for k in ("candidate_pairs", "gjk_calls", "gjk_iterations", "contacts", "wall_contacts",
+                               "fp64_refines", "max_candidates", "sort_passes", "out_of_domain")}
+
+
+def oracle_rate(sc, seconds, threads=1):
+    """FP64 CPU oracle on scene sc for about `seconds`: particle-steps/s."""
     from oracle import pyoracle
-    sc = S.config1(n=8000, seed=7)
     o = S.load(pyoracle.Oracle(dt=sc.dt, threads=threads), sc)
     t0 = time.perf_counter()
     o.step(1)
@@ -175,38 +243,27 @@ def cpu_baseline(seconds, threads=1):
     dt = time.perf_counter() - t0
     st = o.get_stats()
     o.close()
-    return dict(value=sc.n * k / dt, steps=k, seconds=dt, n=sc.n,
-                gjk_pair_tests_per_s=st["gjk_calls"] / dt)
+    return dict(value=sc.n * k / dt, steps=k, seconds=dt, n=sc.n, gjk_pair_tests_per_s=st["gjk_calls"] / dt)
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--fp64-steps", type=int, default=200)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy 1 GiB device to device three times (reference for the HBM byte counters)")
-    args = ap.parse_args()
-    if args.copy_calibration:
-        copy_calibration()
-        return
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the batched multi-simulation form is the way to fill one device", file=sys.stderr)
-        sys.exit(2)
+def contract(metric, value, K, W, ms, dtype, workload, **extra):
+    out = {"metric": metric, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": workload}
+    out.update(extra)
+    return out
 
-    log(f"generating the superquadric bed, n = {args.n}")
+
+def bench_c1(args):
+    K = args.steps or 1000
+    W = args.warmup
+    n = args.n or 1_000_000
+    log(f"generating the superquadric bed, n = {n}")
     t0 = time.perf_counter()
-    sc = S.config1(n=args.n, log=log)
+    sc = S.config1(n=n, log=log)
     log(f"  done in {time.perf_counter() - t0:.1f} s: phi = {sc.notes['phi']:.4f}, "
         f"rounds = {sc.notes['rounds']}, redraws = {sc.notes['redraws']}")
 
-    K, W = args.steps, args.warmup
     log(f"FP32-mixed: {W} warm-up + profiled + {K} timed steps")
     r32 = run_engine(sc, A.FP32_MIXED, W, K)
     ps32 = sc.n * K / r32["wall_s"]
@@ -223,42 +280,9 @@ def main():
     if r64 is not None:
         p32 = run_engine(sc, A.FP32_MIXED, W, K64, profile=False)
         p64 = run_engine(sc, A.FP64, W, K64, profile=False)
-        d = np.abs(p32["pos"] - p64["pos"]).max()
-        disp = np.abs(p64["pos"] - sc.pos).max()
-        err = dict(steps=W + K64, pos_rel=float(d / np.abs(p64["pos"]).max()),
-                   pos_rel_displacement=float(d / max(disp, 1e-300)),
-                   vel_rel=float(np.abs(p32["vel"] - p64["vel"]).max() / max(np.abs(p64["vel"]).max(), 1e-300)),
-                   max_displacement_m=float(disp))
+        err = dict(steps=W + K64, **pos_error(p32, p64, sc.pos))
 
-    st = r32["stats"]
-    nar_ms = r32["phases"]["narrow"]
-    E, C = st["candidate_pairs"] / K, st["contacts"] / K
-    n_shapes = len(np.unique(sc.particle_shape))
-    # compulsory HBM bytes of one k_narrow launch
-    nbytes = sc.n * B_NP_PARTICLE + n_shapes * B_NP_SHAPE + E * B_NP_ENTRY + C * B_NP_CONTACT
-    achieved_gbs = nbytes / (nar_ms * 1e-3) / 1e9
-    # HBM bytes per launch of k_narrow<float>, from the FETCH_SIZE and
-    # WRITE_SIZE counter runs summarised in profiles/ (a line
-    # "hbm_bytes_per_launch = N"); null until measured
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "narrow32_c1_1M_traffic.txt")
-    if os.path.exists(tf):
-        for line in open(tf):
-            if line.strip().startswith("hbm_bytes_per_launch"):
-                traffic = float(line.split("=")[1])
-    roofline = dict(bound="hbm", kernel="k_narrow<float,true,0>", achieved=achieved_gbs, peak=PEAK_HBM_GBS,
-                    unit="GB/s", frac=achieved_gbs / PEAK_HBM_GBS, traffic=traffic,
-                    bytes_per_launch=nbytes, avg_launch_ms=nar_ms,
-                    note="first narrow pass (every candidate entry); compulsory bytes: 64 B pose+ids per "
-                         "particle, 96 B per shape record, 12 B per candidate entry, 24 B per contact; "
-                         "the launch is gather-latency bound (about one GJK iteration per query)")
-    ops = OPS_PER_GJK_ITER * st["gjk_iterations"] / K + OPS_PER_GJK_PAIR * st["gjk_calls"] / K + \
-        OPS_PER_ENTRY * st["candidate_pairs"] / K
-    nar_all_ms = nar_ms + r32["phases"]["narrow_deferred"]
-    achieved = ops / (nar_all_ms * 1e-3) / 1e12
-    roofline_valu = dict(bound="valu", kernel="k_narrow<float,true,0|1>", achieved=achieved, peak=PEAK_FP32_TFLOPS,
-                         unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS, ops_per_launch=ops,
-                         note="no MFMA: no dense contraction on the path; peak = FP32 vector")
+    roofline = narrow_roofline(sc, r32, K, "narrow32_c1_1M_traffic.txt")
     step_ms = r32["wall_s"] * 1e3 / K
     roofline_step = dict(bound="hbm", bytes_per_particle_step=B_PS_FP32,
                          achieved=B_PS_FP32 * ps32 / 1e9, peak=PEAK_HBM_GBS, unit="GB/s",
@@ -274,7 +298,7 @@ def main():
     # velocities (sigma 100 rad/s), W + K steps in both precisions
     agit = None
     if not args.no_fp64:
-        na = min(args.n, 100_000)
+        na = min(n, 100_000)
         sa = S.config1(n=na, seed=11)
         rng = np.random.default_rng(12)
         sa.vel = rng.normal(0.0, 0.1, (na, 3))
@@ -285,17 +309,9 @@ def main():
         # the same FP64 path stopped at the FP32 path's GJK gap: the part of
         # the divergence that the gap alone causes in this chaotic flow
         a64g = run_engine(sa, A.FP64, W, K, profile=False, gjk_tol=GJK_TOL[A.FP32_MIXED])
-        d = np.abs(a32["pos"] - a64["pos"]).max()
-        dg = np.abs(a64g["pos"] - a64["pos"]).max()
-        disp = np.abs(a64["pos"] - sa.pos).max()
-        vmax = np.abs(a64["vel"]).max()
         agit = dict(n=na, steps=W + K, contacts_fp64=a64["stats"]["contacts"],
-                    contacts_fp32=a32["stats"]["contacts"], pos_rel=float(d / np.abs(a64["pos"]).max()),
-                    pos_rel_displacement=float(d / max(disp, 1e-300)), max_displacement_m=float(disp),
-                    vel_rel=float(np.abs(a32["vel"] - a64["vel"]).max() / vmax),
-                    fp64_at_fp32_gap=dict(pos_rel=float(dg / np.abs(a64["pos"]).max()),
-                                          pos_rel_displacement=float(dg / max(disp, 1e-300)),
-                                          vel_rel=float(np.abs(a64g["vel"] - a64["vel"]).max() / vmax)),
+                    contacts_fp32=a32["stats"]["contacts"], **pos_error(a32, a64, sa.pos),
+                    fp64_at_fp32_gap=pos_error(a64g, a64, sa.pos),
                     fp32_particle_steps_per_s=na * K / a32["wall_s"],
                     fp64_particle_steps_per_s=na * K / a64["wall_s"])
         log(f"  error {agit}")
@@ -303,48 +319,243 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline: FP64 oracle, 1 thread, ~{args.cpu_seconds:.0f} s")
-        c = cpu_baseline(args.cpu_seconds, threads=1)
+        c = oracle_rate(S.config1(n=8000, seed=7), args.cpu_seconds)
         cpu = dict(value=c["value"], unit="particle-steps/s", cores=1, kind="port",
                    sample=f"superquadric bed, {c['n']} particles (same distribution and solid fraction), "
                           f"{c['steps']} steps in {c['seconds']:.1f} s",
                    gjk_pair_tests_per_s=c["gjk_pair_tests_per_s"])
         log(f"  {cpu['value']:.3e} particle-steps/s")
 
-    out = {
-        "metric": "particle-steps/s (FP32-mixed) on the superquadric bed",
-        "value": ps32,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": step_ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "c1 superquadric bed: 1e6 superquadrics a,b,c~U[0.4,1] mm, n1,n2~U[2,8], "
-                               "phi=0.3, periodic x,y + floor/ceiling, dt=1e-6, at rest",
-                   "n_particles": sc.n, "phi": sc.notes["phi"], "precision": "FP32-mixed (FP64 state)"},
-        "fp64": None if r64 is None else {
+    st = r32["stats"]
+    out = contract(
+        "particle-steps/s (FP32-mixed) on the superquadric bed", ps32, K, W, step_ms, "f32",
+        {"workload": "c1 superquadric bed: 1e6 superquadrics a,b,c~U[0.4,1] mm, n1,n2~U[2,8], "
+                     "phi=0.3, periodic x,y + floor/ceiling, dt=1e-6, at rest",
+         "n_particles": sc.n, "phi": sc.notes["phi"], "precision": "FP32-mixed (FP64 state)"},
+        fp64=None if r64 is None else {
             "particle_steps_per_s": sc.n * K64 / r64["wall_s"], "ms_per_step": r64["wall_s"] * 1e3 / K64,
             "gjk_pair_tests_per_s": r64["stats"]["gjk_calls"] / r64["wall_s"],
             "speedup_fp32_mixed_over_fp64": ps32 / (sc.n * K64 / r64["wall_s"])},
-        "fp32_mixed_error_vs_fp64": err,
-        "fp32_mixed_error_vs_fp64_agitated": agit,
-        "gjk_pair_tests_per_s": st["gjk_calls"] / r32["wall_s"],
-        "step_stats": {k: st[k] for k in ("candidate_pairs", "gjk_calls", "gjk_iterations", "contacts",
-                                          "wall_contacts", "fp64_refines", "max_candidates", "sort_passes",
-                                          "out_of_domain")},
-        "phase_ms_per_step": r32["phases"],
-        "gjk_microbench": {"fp64": mb64, "fp32": mb32},
-        "roofline": roofline,
-        "roofline_valu": roofline_valu,
-        "roofline_step_hbm": roofline_step,
-        "cpu_baseline": cpu,
-    }
+        fp32_mixed_error_vs_fp64=err,
+        fp32_mixed_error_vs_fp64_agitated=agit,
+        gjk_pair_tests_per_s=st["gjk_calls"] / r32["wall_s"],
+        step_stats=step_stats(st),
+        phase_ms_per_step=r32["phases"],
+        gjk_microbench={"fp64": mb64, "fp32": mb32},
+        roofline=roofline,
+        roofline_valu=valu_roofline(r32, K),
+        roofline_step_hbm=roofline_step,
+        cpu_baseline=cpu)
     if cpu:
         out["speedup_vs_cpu"] = ps32 / cpu["value"]
+    return out
+
+
+def bench_c2(args):
+    """Polydisperse polyhedra: 250k cubes, 8-vertex hulls and spheres (size
+    ratio 3:1), Gaussian velocities, phi ~0.45, walled, dt = 5e-7, 2000
+    steps, FP32-mixed, with FP64 beside it."""
+    K = args.steps or 2000
+    W = args.warmup
+    n = args.n or 250_000
+    log(f"generating the polyhedron bed, n = {n}")
+    t0 = time.perf_counter()
+    sc = S.config2(n=n, log=log)
+    log(f"  done in {time.perf_counter() - t0:.1f} s: phi = {sc.notes['phi']:.4f}, rounds = {sc.notes['rounds']}")
+    tol = GJK_TOL
+    log(f"FP32-mixed: {W} warm-up + profiled + {K} timed steps")
+    r32 = run_engine(sc, A.FP32_MIXED, W, K, max_neighbors=96)
+    ps32 = sc.n * K / r32["wall_s"]
+    log(f"  {r32['wall_s'] * 1e3 / K:.3f} ms/step, {ps32:.3e} particle-steps/s")
+    r64 = None
+    if not args.no_fp64:
+        log(f"FP64: {W} warm-up + profiled + {K} timed steps")
+        r64 = run_engine(sc, A.FP64, W, K, max_neighbors=96)
+        log(f"  {r64['wall_s'] * 1e3 / K:.3f} ms/step")
+        g64 = run_engine(sc, A.FP64, W, K, profile=False, gjk_tol=tol[A.FP32_MIXED], max_neighbors=96)
+        err = dict(steps=W + K, **pos_error(r32, r64, sc.pos), fp64_at_fp32_gap=pos_error(g64, r64, sc.pos))
+    cpu = None
+    if not args.no_cpu:
+        sa = S.config2(n=8000, seed=7)
+        c = oracle_rate(sa, args.cpu_seconds)
+        cpu = dict(value=c["value"], unit="particle-steps/s", cores=1, kind="port",
+                   sample=f"polyhedron bed, {c['n']} particles (same distributions), {c['steps']} steps "
+                          f"in {c['seconds']:.1f} s", gjk_pair_tests_per_s=c["gjk_pair_tests_per_s"])
+    st = r32["stats"]
+    return contract(
+        "particle-steps/s (FP32-mixed) on the polyhedron bed", ps32, K, W, r32["wall_s"] * 1e3 / K, "f32",
+        {"workload": "c2 polydisperse polyhedra: 50% cubes, 30% 8-vertex hulls, 20% spheres, size ratio 3:1 "
+                     "log-uniform, v~N(0,0.1 m/s), walled, dt=5e-7",
+         "n_particles": sc.n, "phi": sc.notes["phi"], "precision": "FP32-mixed (FP64 state)"},
+        fp64=None if r64 is None else {
+            "particle_steps_per_s": sc.n * K / r64["wall_s"], "ms_per_step": r64["wall_s"] * 1e3 / K,
+            "speedup_fp32_mixed_over_fp64": r64["wall_s"] / r32["wall_s"]},
+        fp32_mixed_error_vs_fp64=None if r64 is None else err,
+        gjk_pair_tests_per_s=st["gjk_calls"] / r32["wall_s"],
+        gjk_iterations_per_test=st["gjk_iterations"] / max(st["gjk_calls"], 1),
+        step_stats=step_stats(st), phase_ms_per_step=r32["phases"],
+        roofline=narrow_roofline(sc, r32, K), roofline_valu=valu_roofline(r32, K), cpu_baseline=cpu,
+        speedup_vs_cpu=None if cpu is None else ps32 / cpu["value"])
+
+
+def bench_c3(args):
+    """Batched ensemble: 512 superquadric beds of 500..20,000 particles in one
+    context, every step of all of them replayed from one captured graph; cell
+    size, block size, narrow-phase residency and pass structure chosen by the
+    autotuner on the ensemble itself."""
+    from grains3dgpu_amd.autotune import autotune
+    from grains3dgpu_amd.engine import Engine
+    K = args.steps or 1000
+    W = args.warmup
+    log("generating the ensemble (512 beds)")
+    t0 = time.perf_counter()
+    sc = S.config3(log=None)
+    log(f"  done in {time.perf_counter() - t0:.1f} s: {sc.n} particles, rounds = {sc.notes['rounds']}")
+    t0 = time.perf_counter()
+    tuned = autotune(sc, precision=A.FP32_MIXED, gjk_rel_tol=GJK_TOL[A.FP32_MIXED], steps=10, log=log)
+    tune_s = time.perf_counter() - t0
+    log(f"  autotuner: {tuned['cell_size_factor']}, {tuned['block_size']}, {tuned['narrow_blocks_per_cu']}, "
+        f"{tuned['narrow_passes']} in {tune_s:.1f} s")
+    launch = dict(block_size=tuned["block_size"], narrow_blocks_per_cu=tuned["narrow_blocks_per_cu"],
+                  narrow_passes=tuned["narrow_passes"])
+    r32 = run_engine(sc, A.FP32_MIXED, W, K, **launch)
+    ps32 = sc.n * K / r32["wall_s"]
+    log(f"  {r32['wall_s'] * 1e3 / K:.3f} ms/step, {ps32:.3e} particle-steps/s")
+    # the untuned default launch beside it
+    d32 = run_engine(sc, A.FP32_MIXED, W, min(K, 200), profile=False)
+    cpu = None
+    if not args.no_cpu:
+        k = int(np.argmin(np.abs(np.array(sc.notes["sizes"]) - 5000)))
+        one = S.config3(n_sims=1, n_range=(sc.notes["sizes"][k], sc.notes["sizes"][k] + 1), seed=3 + k)
+        c = oracle_rate(one, args.cpu_seconds)
+        cpu = dict(value=c["value"], unit="particle-steps/s", cores=1, kind="port",
+                   sample=f"one bed of the ensemble's distribution, {c['n']} particles, {c['steps']} steps in "
+                          f"{c['seconds']:.1f} s")
+    del Engine
+    return contract(
+        "particle-steps/s (FP32-mixed) of the batched ensemble", ps32, K, W, r32["wall_s"] * 1e3 / K, "f32",
+        {"workload": "c3 ensemble: 512 superquadric beds, sizes log-uniform in [500, 20000], kn~U[5e4,5e5], "
+                     "mu~U[0.1,0.5], each its own seed, one context, one graph",
+         "n_particles": sc.n, "n_sims": len(sc.sims), "precision": "FP32-mixed (FP64 state)"},
+        autotune=dict(tuned, seconds=tune_s),
+        default_launch_particle_steps_per_s=sc.n * min(K, 200) / d32["wall_s"],
+        step_stats=step_stats(r32["stats"]), phase_ms_per_step=r32["phases"],
+        gjk_pair_tests_per_s=r32["stats"]["gjk_calls"] / r32["wall_s"],
+        roofline=narrow_roofline(sc, r32, K), roofline_valu=valu_roofline(r32, K), cpu_baseline=cpu,
+        speedup_vs_cpu=None if cpu is None else ps32 / cpu["value"])
+
+
+def bench_c4(args):
+    """Differentiable calibration: 8,192 spheres and ellipsoids settle for
+    2,000 steps; loss = relative squared errors of the bed height and mean
+    kinetic energy against a target run (kn = 2e5, e = 0.7, mu = 0.4);
+    gradients with respect to (kn, e, mu) of each of the three contact-table
+    entries through the adjoint (checkpoints every 100 steps), against
+    central finite differences, in FP64 and FP32-mixed."""
+    import torch
+    from grains3dgpu_amd import adjoint
+    from grains3dgpu_amd.engine import Engine
+    K = args.steps or 2000
+    n = args.n or 8192
+    every = 100
+    log(f"generating the calibration bed, n = {n}")
+    sc = S.config4(n=n, log=None)
+    mass = adjoint.masses(sc)
+    p0 = np.array(sc.tables)[:, [0, 1, 3]].copy()
+    pt = np.tile([2e5, 0.7, 0.4], (p0.shape[0], 1))
+    res = {}
+    for prec, name in ((A.FP64, "fp64"), (A.FP32_MIXED, "fp32_mixed")):
+        e = S.load(Engine(dt=sc.dt, precision=prec, gjk_rel_tol=GJK_TOL[prec]), sc)
+
+        def final(params):
+            s2 = adjoint.with_params(sc, params)
+            e.set_contact_tables(s2)
+            e.set_state(s2.pos, s2.quat, s2.vel, s2.omega)
+            e.step(K, graph=True)
+            p, _, v, _ = e.get_state()
+            return p, v
+
+        def loss_of(p, v, target):
+            return adjoint.bed_loss(torch.from_numpy(p), torch.from_numpy(v), torch.from_numpy(mass), target)[0].item()
+
+        pT, vT = final(pt)
+        target = (float(pT[:, 2].mean()), float((0.5 * mass * (vT * vT).sum(1)).mean()))
+        t0 = time.perf_counter()
+        final(p0)
+        fwd_plain = time.perf_counter() - t0
+        params = torch.tensor(p0, requires_grad=True)
+        t0 = time.perf_counter()
+        pos, quat, vel, om = adjoint.simulate(e, sc, params, K, every)
+        fwd = time.perf_counter() - t0
+        L, h, ke = adjoint.bed_loss(pos, vel, torch.from_numpy(mass), target)
+        t0 = time.perf_counter()
+        L.backward()
+        bwd = time.perf_counter() - t0
+        g = params.grad.numpy().copy()
+        fds = {}
+        for rel in (1e-6, 1e-4):
+            fd = np.zeros_like(p0)
+            for r in range(p0.shape[0]):
+                for c in range(3):
+                    hstep = rel * p0[r, c]
+                    pp, pm = p0.copy(), p0.copy()
+                    pp[r, c] += hstep
+                    pm[r, c] -= hstep
+                    fd[r, c] = (loss_of(*final(pp), target) - loss_of(*final(pm), target)) / (2 * hstep)
+            fds[rel] = fd
+            log(f"  {name}: FD {rel:g} done")
+        scale = np.abs(g).max(axis=1, keepdims=True)
+        res[name] = dict(
+            loss=float(L), bed_height=float(h), mean_ke=float(ke), target=target,
+            grad=g.tolist(), fd_1e_6=fds[1e-6].tolist(), fd_1e_4=fds[1e-4].tolist(),
+            rel_err_vs_fd_1e_6=float((np.abs(g - fds[1e-6]) / scale).max()),
+            rel_err_vs_fd_1e_4=float((np.abs(g - fds[1e-4]) / scale).max()),
+            forward_ms_per_step=fwd_plain * 1e3 / K, recorded_forward_ms_per_step=fwd * 1e3 / K,
+            adjoint_ms_per_step=bwd * 1e3 / K, contacts=e.get_stats()["contacts"])
+        log(f"  {name}: loss {float(L):.4e}, grad {g.tolist()}, fd(1e-4) {fds[1e-4].tolist()}")
+        e.close()
+    g64, g32 = np.array(res["fp64"]["grad"]), np.array(res["fp32_mixed"]["grad"])
+    res["fp32_mixed"]["grad_rel_diff_vs_fp64"] = float((np.abs(g32 - g64) / np.abs(g64).max(1, keepdims=True)).max())
+    r = res["fp64"]
+    total = (r["recorded_forward_ms_per_step"] + r["adjoint_ms_per_step"]) * 1e-3 * K
+    cpu = None
+    if not args.no_cpu:
+        c = oracle_rate(sc, args.cpu_seconds)
+        cpu = dict(value=c["value"], unit="particle-steps/s", cores=1, kind="port",
+                   sample=f"the same bed, forward only (the oracle has no adjoint), {c['steps']} steps in "
+                          f"{c['seconds']:.1f} s")
+    return contract(
+        "particle-steps/s of forward + adjoint sweep (FP64) on the calibration bed", n * K / total, K, 0,
+        total * 1e3 / K, "f64",
+        {"workload": "c4 calibration: 8192 spheres + ellipsoids (ratio <= 2:1), phi=0.35, walled, "
+                     "2000 steps at dt=1e-6, checkpoints every 100, gradients wrt kn, e, mu of 3 table entries",
+         "n_particles": n, "checkpoint_every": every},
+        fp64=res["fp64"], fp32_mixed=res["fp32_mixed"], roofline=None, cpu_baseline=cpu)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's)")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=None, help="particles (default: the workload's)")
+    ap.add_argument("--workload", default="c1", choices=("c1", "c2", "c3", "c4"),
+                    help="c1 superquadric bed (default), c2 polyhedra, c3 batched ensemble, c4 adjoint")
+    ap.add_argument("--fp64-steps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (reference for the HBM byte counters)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the batched multi-simulation form is the way to fill one device", file=sys.stderr)
+        sys.exit(2)
+    out = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -18,6 +18,7 @@ only the autograd plumbing here.
 import numpy as np
 import torch
 
+from . import _abi as A
 from . import scene as S
 
 
@@ -63,6 +64,19 @@ def simulate(engine, scene, params, n_steps, checkpoint_every=100, graph=True):
     """(pos, quat, vel, omega) after n_steps from scene's state with contact
     parameters `params` (entries x 3: kn, en, muc), differentiable in params."""
     return Trajectory.apply(params, engine, scene, n_steps, checkpoint_every, graph)
+
+
+def masses(scene):
+    """Particle masses of a scene of spheres and ellipsoids (superquadrics
+    with n1 = n2 = 2), the shapes the adjoint covers: rho * 4/3 pi r^3 or
+    rho * 4/3 pi a b c."""
+    out = np.zeros(scene.n)
+    for i, s in enumerate(scene.particle_shape):
+        sh = scene.shapes[int(s)]
+        d = sh["dims"]
+        v = 4.0 / 3.0 * np.pi * (d[0] ** 3 if int(sh["kind"]) == A.SPHERE else d[0] * d[1] * d[2])
+        out[i] = sh["density"] * v
+    return out
 
 
 def bed_loss(pos, vel, mass, target):

@@ -241,22 +241,35 @@ void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx
 }
 
 // ---- cell ranges and sorted records -----------------------------------------------
-// One thread per sorted position s: the CSR offsets of the cells (cell_off[c]
-// = first sorted position whose key is >= c; thread s writes the cells in
-// (key[s-1], key[s]], the last thread also those after its key, so every one
-// of the C + 1 offsets is written once and empty cells get empty ranges), the
-// inverse permutation, and the particle's records in cell order -- the only
-// gather of the state per step; the candidate and narrow phases read bp / pk
-// of neighbouring cells, which are neighbouring lines.
+// The CSR offsets of the cells: cell_off[c] = first sorted position whose key
+// is >= c (a lower bound in the sorted keys), one thread per offset, so every
+// one of the C + 1 offsets is written once, coalesced, and empty cells get
+// empty ranges.  (Filling the empty cells from the thread of the particle
+// before them serialises the empty layers at the floor and the ceiling of a
+// bed on one thread each.)
+__global__ void k_cell_off(const uint32_t* __restrict__ ks, uint64_t n, uint64_t ncells,
+                           uint32_t* __restrict__ off) {
+  const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncells) return;
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)ks[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  off[c] = (uint32_t)lo;
+}
+
+// One thread per sorted position s: the inverse permutation and the
+// particle's records in cell order -- the only gather of the state per step;
+// the candidate and narrow phases read bp / pk of neighbouring cells, which
+// are neighbouring lines.
 __global__ void k_cells(DevScene d, const uint32_t* __restrict__ ks, const uint32_t* __restrict__ is) {
   const uint64_t n = d.n;
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const uint32_t k = ks[s];
-  const uint32_t c0 = s == 0 ? 0u : ks[s - 1] + 1u;
-  for (uint32_t c = c0; c <= k; ++c) d.cell_off[c] = (uint32_t)s;
-  if (s == n - 1)
-    for (uint32_t c = k + 1u; c <= d.ncells; ++c) d.cell_off[c] = (uint32_t)n;
   const uint32_t i = is[s];
   d.rank[i] = (uint32_t)s;
   const uint32_t shp = d.shape[i];
@@ -275,10 +288,10 @@ __global__ void k_cells(DevScene d, const uint32_t* __restrict__ ks, const uint3
 }
 
 void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
-  if (d.n == 0) {
-    launch_fill_u32(d.cell_off, (uint64_t)d.ncells + 1, 0u, s);
-    return;
-  }
+  const uint64_t offs = (uint64_t)d.ncells + 1;
+  hipLaunchKernelGGL(k_cell_off, dim3((unsigned)((offs + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, keys_s, d.n,
+                     (uint64_t)d.ncells, d.cell_off);
+  if (d.n == 0) return;
   hipLaunchKernelGGL(k_cells, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d, keys_s,
                      idx_s);
 }
@@ -289,9 +302,9 @@ void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx
 // cand[k * N + s] (k-major, so the lanes of a wave write neighbouring words)
 // and their count into cand_count[s].  Residents of a cell ascend in sorted
 // position exactly as in particle index (the sort is stable on an identity
-// payload), and the three cells x-1, x, x+1 of a neighbour row have
-// consecutive keys, so away from the x borders a row is one range
-// [off[c-1], off[c+2]) visited in the reference order.
+// payload).  Each neighbour cell is its own loop: merging the three cells of
+// a row into one range [off[c-1], off[c+2]) (they have consecutive keys) was
+// measured 24 % slower on the 10^6 superquadric bed.
 __device__ __forceinline__ void cand_scan(const DevScene& d, uint32_t t0, uint32_t t1, uint32_t s, V3<double> xi,
                                           double ri, const SimGrid& g, uint32_t& cnt, uint32_t& over) {
   const uint64_t n = d.n;
@@ -325,7 +338,6 @@ __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_
     const int cy = (int)((local / (uint32_t)nx) % (uint32_t)ny);
     const int cz = (int)(local / ((uint32_t)nx * (uint32_t)ny));
     const bool px = g.periodic & 1, py = g.periodic & 2, pz = g.periodic & 4;
-    const bool row_ok = cx >= 1 && cx + 1 < nx;  // x-1 .. x+1 inside, no wrap
     for (int dz = -1; dz <= 1; ++dz) {
       int z = cz + dz;
       if (pz) z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
@@ -335,16 +347,12 @@ __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_
         if (py) y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
         if (y < 0 || y >= ny) continue;
         const uint32_t row = g.base + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
-        if (row_ok) {
-          cand_scan(d, d.cell_off[row + cx - 1], d.cell_off[row + cx + 2], (uint32_t)s, xi, ri, g, cnt, over);
-        } else {
-          for (int dx = -1; dx <= 1; ++dx) {
-            int x = cx + dx;
-            if (px) x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
-            if (x < 0 || x >= nx) continue;
-            const uint32_t c = row + (uint32_t)x;
-            cand_scan(d, d.cell_off[c], d.cell_off[c + 1], (uint32_t)s, xi, ri, g, cnt, over);
-          }
+        for (int dx = -1; dx <= 1; ++dx) {
+          int x = cx + dx;
+          if (px) x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+          if (x < 0 || x >= nx) continue;
+          const uint32_t c = row + (uint32_t)x;
+          cand_scan(d, d.cell_off[c], d.cell_off[c + 1], (uint32_t)s, xi, ri, g, cnt, over);
         }
       }
     }

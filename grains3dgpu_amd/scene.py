@@ -529,3 +529,145 @@ def config4(n=8192, phi=0.35, seed=4, kn=1e5, en=0.8, mu=0.3, vel=0.1, crust_fra
     vol = np.where(sq, 4.0 / 3.0 * np.pi * dims[:, 0] * dims[:, 1] * dims[:, 2], 4.0 / 3.0 * np.pi * dims[:, 0] ** 3)
     sc.notes = dict(phi=float(vol.sum() / (L - 2 * margin) ** 3), rounds=rounds, steps=2000)
     return sc
+
+
+# ---------------------------------------------------------------------------
+# polydisperse convex polyhedra (config 2)
+def config2(n=250_000, phi=0.45, seed=2, r_rng=(2e-3, 6e-3), shares=(0.5, 0.3, 0.2), vel=0.1,
+            crust_frac=0.05, kn=1e5, en=0.8, mu=0.3, tilt_deg=5.0, jitter=0.02, max_rounds=80, precision=None,
+            log=None, resolve=True):
+    """Polydisperse convex polyhedra in a walled box: shares of cubes (Box),
+    8-vertex convex hulls of points uniform on a sphere (Polyhedron, each
+    particle its own hull, centred on its volume centroid) and spheres.  A
+    size r, log-uniform in r_rng (ratio 3:1), is the cube's half edge, the
+    radius of the hull's sphere and the sphere's radius.  rho = 2500,
+    Gaussian velocities (sigma `vel` m/s per axis), gravity -9.81 z,
+    dt = 5e-7 s, Hooke kn = 1e5, en = 0.8, mu = 0.3 (as config 0), the
+    tangential damping matched to the normal one.
+
+    Placement: particles sorted by size, largest first, fill horizontal
+    lattice layers from the floor; each layer's spacing comes from its own
+    mean volume and phi, so the bed reaches phi although sizes span 3:1 (one
+    spacing for all would leave the large particles overlapping and the
+    small ones loose).  At phi = 0.45 the spacing is ~2.25 r: a cube (edge
+    2 r) fits only when nearly aligned with the lattice, so cubes start
+    tilted by at most `tilt_deg` about a random axis (the velocities mix
+    them within a few collisions); hulls and spheres are uniformly random.
+    Remaining overlaps of the actual shapes are removed with the library's
+    own narrow phase -- the higher index of each overlapping pair gets a new
+    orientation, from round 6 also its size shrunk by 0.95.  The solid
+    fraction reached (over the lattice region) is notes['phi']."""
+    from .shapes import random_hull_points_batch, hull_volume_centroid_batch
+    rng = np.random.default_rng(seed)
+    counts = [int(round(s * n)) for s in shares[:2]]
+    kind = np.full(n, A.SPHERE, np.int32)
+    kind[:counts[0]] = A.BOX
+    kind[counts[0]:counts[0] + counts[1]] = A.POLYHEDRON
+    kind = kind[rng.permutation(n)]
+    r = np.exp(rng.uniform(math.log(r_rng[0]), math.log(r_rng[1]), n))
+    hull = kind == A.POLYHEDRON
+    unit = random_hull_points_batch(rng, int(hull.sum()))  # unit sphere, centroid at 0
+    uvol, _ = hull_volume_centroid_batch(unit)
+    hull_row = np.full(n, -1)
+    hull_row[hull] = np.arange(hull.sum())
+    unit_rc = np.linalg.norm(unit, axis=2).max(1)
+
+    def vol_rc(rr):
+        v = np.where(kind == A.BOX, 8.0 * rr ** 3, 4.0 / 3.0 * np.pi * rr ** 3)
+        rc = np.where(kind == A.BOX, math.sqrt(3.0) * rr, rr)
+        v[hull] = uvol * rr[hull] ** 3
+        rc[hull] = unit_rc * rr[hull]
+        return v, rc
+
+    vol, rc = vol_rc(r)
+    Lxy = (vol.sum() / phi) ** (1.0 / 3.0)
+    order = np.argsort(-r, kind="stable")
+    pos = np.zeros((n, 3))
+    z, i, pad = 0.0, 0, 0.0
+    while i < n:
+        s = (vol[order[i]] / phi) ** (1.0 / 3.0)
+        for _ in range(3):  # the layer's spacing from its own mean volume
+            row = max(1, int(round(Lxy / s)))
+            m = min(row * row, n - i)
+            s = (vol[order[i:i + m]].mean() / phi) ** (1.0 / 3.0)
+        row = max(1, int(round(Lxy / s)))
+        m = min(row * row, n - i)
+        sp = Lxy / row
+        k = np.arange(m)
+        cy, cx = np.divmod(k, row)
+        idx = order[i:i + m]
+        pos[idx, 0] = (cx + 0.5) * sp
+        pos[idx, 1] = (cy + 0.5) * sp
+        pos[idx, 2] = z + 0.5 * s
+        pos[idx] += rng.uniform(-jitter, jitter, (m, 3)) * s
+        pad = max(pad, float((rc[idx] - 0.45 * min(s, sp)).max()))
+        z += s
+        i += m
+    lo = np.array([-pad, -pad, -pad])
+    hi = np.array([Lxy + pad, Lxy + pad, z + pad])
+    box = kind == A.BOX
+
+    def orient(sel):
+        q = random_quaternions(rng, len(sel))
+        cube = box[sel]
+        ax = rng.standard_normal((int(cube.sum()), 3))
+        ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+        half = 0.5 * np.radians(tilt_deg) * rng.uniform(0.0, 1.0, len(ax))
+        q[cube] = np.concatenate([ax * np.sin(half)[:, None], np.cos(half)[:, None]], 1)
+        return q
+
+    quat = orient(np.arange(n))
+    rho = 2500.0
+    m0 = rho * float(vol.mean())
+    etat = matched_etat(kn, en, 0.5 * m0)
+    tables = np.array([hooke_row(kn, en, etat, mu)])
+    velocity = rng.normal(0.0, vel, (n, 3))
+
+    def make():
+        sh = np.zeros(n, SHAPE_DTYPE)
+        sh["kind"] = kind
+        sh["density"] = rho
+        sh["crust"] = crust_frac * r
+        sh["dims"][:, 0] = r
+        sh["dims"][box, 1] = r[box]
+        sh["dims"][box, 2] = r[box]
+        sh["n_vertices"][hull] = 8
+        sh["vertex_offset"][hull] = 8 * hull_row[hull]
+        verts = (unit * r[hull][:, None, None]).reshape(-1, 3)
+        ws, otr = box_walls(lo, hi)
+        wrec = np.zeros(len(ws), SHAPE_DTYPE)
+        for w, s in zip(wrec, ws):
+            w["kind"] = s["kind"]
+            w["dims"][:2] = s["dims"]
+        return Scene(shapes=np.concatenate([sh, wrec]), tables=tables, n_materials=1,
+                     sims=[dict(domain_min=tuple(lo), domain_max=tuple(hi), gravity=(0, 0, -9.81))],
+                     particle_shape=np.arange(n, dtype=np.uint32), pos=pos.copy(), quat=quat.copy(),
+                     vel=velocity.copy(), omega=np.zeros((n, 3)),
+                     obstacle_shape=np.arange(n, n + len(ws), dtype=np.uint32), obstacle_tr=otr,
+                     vertices=verts, dt=5e-7, name=f"c2_polyhedra_{n}")
+
+    rounds = 0
+    if resolve:
+        from .engine import Engine
+        prec = A.FP32_MIXED if precision is None else precision
+        for rounds in range(1, max_rounds + 1):
+            sc = make()
+            eng = load(Engine(dt=sc.dt, precision=prec, max_neighbors=96), sc)
+            eng.step(1)
+            pairs = eng.get_pairs(A.PAIRS_CONTACT)
+            eng.close()
+            bad = np.unique(np.maximum(pairs[:, 0], pairs[:, 1])) if len(pairs) else np.zeros(0, int)
+            if log:
+                log(f"config2 round {rounds}: {len(pairs) // 2} overlapping pairs")
+            if len(bad) == 0:
+                break
+            quat[bad] = orient(bad)
+            if rounds >= 6:
+                r[bad] *= 0.95
+        else:
+            raise RuntimeError("config2: overlaps remain")
+    sc = make()
+    vol, _ = vol_rc(r)
+    sc.notes = dict(phi=float(vol.sum() / (Lxy * Lxy * z)), rounds=rounds, steps=2000, etat=etat,
+                    box=(lo.tolist(), hi.tolist()))
+    return sc

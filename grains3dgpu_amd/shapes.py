@@ -84,6 +84,48 @@ def random_hull_points(rng, n_points=8, radius=1.0):
     return p - c
 
 
+def random_hull_points_batch(rng, n_hulls, n_points=8):
+    """n_hulls sets of n_points points uniform on the unit sphere, each
+    shifted so that the volume centroid of its convex hull is the origin:
+    (n_hulls, n_points, 3)."""
+    p = rng.standard_normal((n_hulls, n_points, 3))
+    p /= np.linalg.norm(p, axis=2, keepdims=True)
+    return p - hull_centroid_batch(p)[:, None, :]
+
+
+def hull_centroid_batch(p):
+    """Volume centroids of the convex hulls of p (H, n, 3), points in
+    general position: a triple of points spans a facet when every other
+    point lies on one side of its plane; the facet, oriented outwards, adds
+    the signed tetrahedron it forms with the origin."""
+    return hull_volume_centroid_batch(p)[1]
+
+
+def hull_volume_centroid_batch(p):
+    """(volumes (H,), centroids (H, 3)) of the convex hulls of p (H, n, 3)."""
+    H, n, _ = p.shape
+    V = np.zeros(H)
+    c = np.zeros((H, 3))
+    for i in range(n):
+        for j in range(i + 1, n):
+            for k in range(j + 1, n):
+                a, b, cc = p[:, i], p[:, j], p[:, k]
+                nrm = np.cross(b - a, cc - a)
+                s = np.einsum("hpk,hk->hp", p - a[:, None, :], nrm)
+                s[:, [i, j, k]] = 0.0
+                tol = 1e-14 * np.linalg.norm(nrm, axis=1)[:, None]
+                pos = (s > tol).any(1)
+                neg = (s < -tol).any(1)
+                facet = ~(pos & neg)
+                # outward orientation: the other points lie on the negative side
+                b2 = np.where(pos[:, None], cc, b)
+                c2 = np.where(pos[:, None], b, cc)
+                d = np.einsum("hk,hk->h", a, np.cross(b2, c2)) / 6.0 * facet
+                V += d
+                c += d[:, None] * (a + b2 + c2) / 4.0
+    return V, c / V[:, None]
+
+
 def hull_centroid(p):
     n = len(p)
     V = 0.0

@@ -24,7 +24,22 @@ STEPS, EVERY = 150, 40
 
 @pytest.fixture(scope="module")
 def bed():
-    return S.config4(n=64, seed=5, vel=0.2)
+    """The floor (the first wall) is raised to 5 um above the lowest point of
+    any particle, so that wall contacts are in the differentiated steps."""
+    sc = S.config4(n=64, seed=5, vel=0.5)  # all three table entries see contacts in 150 steps
+    R = S.quat_to_matrix(sc.quat).reshape(-1, 3, 3)
+    low = np.zeros(sc.n)
+    for i, k in enumerate(sc.particle_shape):
+        sh = sc.shapes[int(k)]
+        d = np.asarray(sh["dims"], float)
+        # extent along z: r for a sphere, |diag(a, b, c) R^T e_z| for an ellipsoid
+        ext = d[0] if int(sh["kind"]) == A.SPHERE else float(np.linalg.norm(d[:3] * R[i, 2, :]))
+        low[i] = sc.pos[i, 2] - ext
+    tr = np.array(sc.obstacle_tr, copy=True)
+    assert tr[0, 11] == sc.sims[0]["domain_min"][2]  # the floor: normal +z, origin on z = min
+    tr[0, 11] = low.min() + 5e-6
+    sc.obstacle_tr = tr
+    return sc
 
 
 def run(sc, params, precision=A.FP64):
