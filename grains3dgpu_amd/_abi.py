@@ -42,7 +42,7 @@ class Config(C.Structure):
                 ("flags", C.c_uint32), ("gjk_rel_tol", C.c_double),
                 ("max_neighbors", C.c_int32), ("block_size", C.c_int32),
                 ("narrow_blocks_per_cu", C.c_int32), ("narrow_passes", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("candidate_staging", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class Stats(C.Structure):

@@ -9,9 +9,11 @@
 //   k_cells       CSR cell offsets from the sorted keys (the cell start/end of
 //                 ComponentManagerGPU_Kernels.cpp:28-75), the inverse
 //                 permutation and the pose records in cell order
-//   k_candidates  per particle, the 27 cells in the reference order
+//   k_candidates_direct / _staged
+//                 per particle, the 27 cells in the reference order
 //                 (LinkedCell.cpp:41-70) and their residents ascending
-//                 (ComponentManagerCPU.cpp:364-421), bounding-sphere prefilter
+//                 (ComponentManagerCPU.cpp:364-421), bounding-sphere prefilter;
+//                 the second stages the block's neighbour windows in LDS
 //   scan/expand   CSR offsets and the flat (i, j) entry list, in sorted
 //                 positions
 #include "gr_internal.hpp"
@@ -284,7 +286,10 @@ __global__ void k_cells(DevScene d, const uint32_t* __restrict__ ks, const uint3
   r.shape = shp;
   r.sim = d.sim[i];
   d.pk[s] = r;
-  d.bp[s] = make_double4(r.x, r.y, r.z, d.rc_pf[shp]);
+  const double rc = d.rc_pf[shp];
+  d.bp[s] = make_double4(r.x, r.y, r.z, rc);
+  const SimGrid& g = d.grids[r.sim];
+  d.bp32[s] = make_float4((float)(r.x - g.lo[0]), (float)(r.y - g.lo[1]), (float)(r.z - g.lo[2]), (float)rc);
 }
 
 void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
@@ -323,39 +328,195 @@ __device__ __forceinline__ void cand_scan(const DevScene& d, uint32_t t0, uint32
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_t* __restrict__ ks) {
+// The 27 neighbour cells of cell `key` in the reference order (LinkedCell.cpp:
+// 41-70: z outer, then y, then x): f(w, c) for each cell c inside the grid,
+// periodic axes wrapped; w = (dz + 1) * 3 + (dy + 1) is the neighbour row.
+template <class F>
+__device__ __forceinline__ void for_each_neighbor(const SimGrid& g, uint32_t key, F&& f) {
+  const int nx = (int)g.n3[0], ny = (int)g.n3[1], nz = (int)g.n3[2];
+  const uint32_t local = key - g.base;
+  const int cx = (int)(local % (uint32_t)nx);
+  const int cy = (int)((local / (uint32_t)nx) % (uint32_t)ny);
+  const int cz = (int)(local / ((uint32_t)nx * (uint32_t)ny));
+  const bool px = g.periodic & 1, py = g.periodic & 2, pz = g.periodic & 4;
+  for (int dz = -1; dz <= 1; ++dz) {
+    int z = cz + dz;
+    if (pz) z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+    if (z < 0 || z >= nz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int y = cy + dy;
+      if (py) y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+      if (y < 0 || y >= ny) continue;
+      const int w = (dz + 1) * 3 + (dy + 1);
+      const uint32_t row = g.base + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+      for (int dx = -1; dx <= 1; ++dx) {
+        int x = cx + dx;
+        if (px) x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        if (x < 0 || x >= nx) continue;
+        f(w, row + (uint32_t)x);
+      }
+    }
+  }
+}
+
+// Every lane reads its neighbours' records from global memory (lanes of a
+// cell read the same lines, which the caches serve).  The resident checks --
+// ~200 per particle, nearly all rejected -- read the 16-byte FP32 records
+// bp32 (positions relative to the sim's domain minimum) with a conservative
+// test; every pair it passes is decided by the FP64 test of cand_scan, so
+// the candidate lists are those of the FP64 prefilter bit for bit.  The
+// relative coordinates carry at most 2^-24 of their magnitude, the
+// difference and the periodic shift as much again of theirs; the margin is
+// 1e-6 of those magnitudes, several times the bound.
+__global__ void __launch_bounds__(kBlock) k_candidates_direct(DevScene d, const uint32_t* __restrict__ ks) {
   const uint64_t n = d.n;
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cnt = 0, over = 0;
+  if (s < n) {
+    const double4 bi = d.bp[s];
+    const float4 fi = d.bp32[s];
+    const SimGrid g = d.grids[d.pk[s].sim];
+    const V3<double> xi = {bi.x, bi.y, bi.z};
+    const bool px = g.periodic & 1, py = g.periodic & 2, pz = g.periodic & 4;
+    const float Lx = (float)g.L[0], Ly = (float)g.L[1], Lz = (float)g.L[2];
+    // periodic image by a reciprocal: it can pick another image than the
+    // FP64 test only at separations near L/2, far beyond any candidate
+    const float iLx = 1.f / Lx, iLy = 1.f / Ly, iLz = 1.f / Lz;
+    const float base = fmaxf(fabsf(fi.x), fmaxf(fabsf(fi.y), fabsf(fi.z))) +
+                       3.f * (g.periodic ? fmaxf(Lx, fmaxf(Ly, Lz)) : 0.f);
+    for_each_neighbor(g, ks[s], [&](int, uint32_t c) {
+      const uint32_t t1 = d.cell_off[c + 1];
+      for (uint32_t t = d.cell_off[c]; t < t1; ++t) {
+        if (t == (uint32_t)s) continue;
+        const float4 r = d.bp32[t];
+        float ex = r.x - fi.x, ey = r.y - fi.y, ez = r.z - fi.z;
+        if (px) ex -= Lx * rintf(ex * iLx);
+        if (py) ey -= Ly * rintf(ey * iLy);
+        if (pz) ez -= Lz * rintf(ez * iLz);
+        const float mj = fmaxf(fabsf(r.x), fmaxf(fabsf(r.y), fabsf(r.z)));
+        const float rs = (fi.w + r.w) * 1.000001f + 1e-6f * (base + mj);
+        if (!(ex * ex + ey * ey + ez * ez < rs * rs)) continue;
+        cand_scan(d, t, t + 1, (uint32_t)s, xi, bi.w, g, cnt, over);
+      }
+    });
+    d.cand_count[s] = cnt < d.K ? cnt : d.K;
+  }
+  count_add(d.counters, kCntOverflow, over);
+  count_max(d.counters, kCntMaxCand, cnt);
+}
+
+// The block's neighbour records are staged in LDS first.  The block holds
+// consecutive sorted positions, so its keys span [kmin, kmax] and the cells
+// its lanes visit in neighbour row (dz, dy) lie in the key window
+// [kmin + o - 1, kmax + o + 1], o = (dz ny + dy) nx: nine windows, each a
+// contiguous range of sorted positions [off[lo], off[hi + 1]), read once,
+// coalesced, instead of once per visiting lane.  Records are kept in FP32
+// relative to the block's first particle (16 B); a lane's cell whose range
+// lies inside its row's window is scanned from LDS with a conservative FP32
+// test, and every pair the FP32 test passes is decided by the FP64 test of
+// cand_scan, so the candidate lists are those of the FP64 prefilter bit for
+// bit.  Cells outside the windows (periodic wraps, a block spanning sims,
+// windows beyond the LDS capacity) are scanned from global memory.  It pays
+// off when cells hold many particles (long scans per lane); with a few per
+// cell the direct kernel, at twice the occupancy, is faster.
+constexpr int kCandStage = 2520;  // staged records per block: 4 blocks per CU in 160 KB of LDS
+
+__global__ void __launch_bounds__(kBlock) k_candidates_staged(DevScene d, const uint32_t* __restrict__ ks) {
+  __shared__ float4 stage[kCandStage];
+  __shared__ uint32_t w_lo[9], w_hi[9], w_base[10];
+  __shared__ uint32_t vmax_bits;
+  __shared__ double org[3];
+  const uint64_t n = d.n;
+  const uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x;
+  const uint64_t s = b0 + threadIdx.x;
+  if (threadIdx.x < 9) {
+    const uint64_t last = b0 + blockDim.x - 1 < n ? b0 + blockDim.x - 1 : n - 1;
+    const SimGrid& g0 = d.grids[d.pk[b0].sim];
+    const int64_t dz = (int64_t)(threadIdx.x / 3) - 1, dy = (int64_t)(threadIdx.x % 3) - 1;
+    const int64_t o = (dz * (int64_t)g0.n3[1] + dy) * (int64_t)g0.n3[0];
+    int64_t lo = (int64_t)ks[b0] + o - 1, hi = (int64_t)ks[last] + o + 1;
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > (int64_t)d.ncells - 1 ? (int64_t)d.ncells - 1 : hi;
+    uint32_t plo = 0, phi = 0;
+    if (lo <= hi) {
+      plo = d.cell_off[lo];
+      phi = d.cell_off[hi + 1];
+    }
+    w_lo[threadIdx.x] = plo;
+    w_hi[threadIdx.x] = phi;
+  }
+  if (threadIdx.x == 0) {
+    vmax_bits = 0u;
+    const double4 q = d.bp[b0];
+    org[0] = q.x;
+    org[1] = q.y;
+    org[2] = q.z;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // greedy: windows in order while they fit
+    uint32_t tot = 0;
+    for (int w = 0; w < 9; ++w) {
+      const uint32_t sz = w_hi[w] - w_lo[w];
+      w_base[w] = tot;
+      if (tot + sz <= (uint32_t)kCandStage)
+        tot += sz;
+      else
+        w_hi[w] = w_lo[w];  // not staged: its cells are read from global memory
+    }
+    w_base[9] = tot;
+  }
+  __syncthreads();
+  {
+    const uint32_t tot = w_base[9];
+    float vm = 0.f;
+    int w = 0;
+    for (uint32_t k = threadIdx.x; k < tot; k += blockDim.x) {
+      while (k >= w_base[w + 1]) ++w;
+      const double4 q = d.bp[w_lo[w] + (k - w_base[w])];
+      const float4 r = make_float4((float)(q.x - org[0]), (float)(q.y - org[1]), (float)(q.z - org[2]), (float)q.w);
+      stage[k] = r;
+      vm = fmaxf(vm, fmaxf(fabsf(r.x), fmaxf(fabsf(r.y), fabsf(r.z))));
+    }
+    atomicMax(&vmax_bits, __float_as_uint(vm));  // non-negative floats order as their bits
+  }
+  __syncthreads();
   uint32_t cnt = 0, over = 0;
   if (s < n) {
     const double4 bi = d.bp[s];
     const SimGrid g = d.grids[d.pk[s].sim];
     const V3<double> xi = {bi.x, bi.y, bi.z};
     const double ri = bi.w;
-    const int nx = (int)g.n3[0], ny = (int)g.n3[1], nz = (int)g.n3[2];
-    const uint32_t local = ks[s] - g.base;
-    const int cx = (int)(local % (uint32_t)nx);
-    const int cy = (int)((local / (uint32_t)nx) % (uint32_t)ny);
-    const int cz = (int)(local / ((uint32_t)nx * (uint32_t)ny));
     const bool px = g.periodic & 1, py = g.periodic & 2, pz = g.periodic & 4;
-    for (int dz = -1; dz <= 1; ++dz) {
-      int z = cz + dz;
-      if (pz) z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
-      if (z < 0 || z >= nz) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        int y = cy + dy;
-        if (py) y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
-        if (y < 0 || y >= ny) continue;
-        const uint32_t row = g.base + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
-        for (int dx = -1; dx <= 1; ++dx) {
-          int x = cx + dx;
-          if (px) x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
-          if (x < 0 || x >= nx) continue;
-          const uint32_t c = row + (uint32_t)x;
-          cand_scan(d, d.cell_off[c], d.cell_off[c + 1], (uint32_t)s, xi, ri, g, cnt, over);
+    // the FP32 test: relative coordinates carry at most 2^-24 of their
+    // magnitude (bounded by the staged maximum and the lane's own), the
+    // difference and the periodic shift as much again of theirs; the margin
+    // is 1e-6 of those magnitudes, several times the bound
+    const float xf = (float)(xi.x - org[0]), yf = (float)(xi.y - org[1]), zf = (float)(xi.z - org[2]);
+    const float Lx = (float)g.L[0], Ly = (float)g.L[1], Lz = (float)g.L[2];
+    const float Lw = g.periodic ? fmaxf(Lx, fmaxf(Ly, Lz)) : 0.f;
+    const float mi = fmaxf(fabsf(xf), fmaxf(fabsf(yf), fabsf(zf)));
+    const float margin = 1e-6f * (__uint_as_float(vmax_bits) + mi + 3.f * Lw);
+    const float rif = (float)ri;
+    for_each_neighbor(g, ks[s], [&](int w, uint32_t c) {
+      const uint32_t wlo = w_lo[w], whi = w_hi[w];
+      const uint32_t t0 = d.cell_off[c], t1 = d.cell_off[c + 1];
+      if (t0 >= wlo && t1 <= whi) {
+        const uint32_t sb = w_base[w] - wlo;  // index t + sb (mod 2^32) = w_base[w] + (t - wlo)
+        for (uint32_t t = t0; t < t1; ++t) {
+          if (t == (uint32_t)s) continue;
+          const float4 r = stage[t + sb];
+          float ex = r.x - xf, ey = r.y - yf, ez = r.z - zf;
+          if (px) ex -= Lx * rintf(ex / Lx);
+          if (py) ey -= Ly * rintf(ey / Ly);
+          if (pz) ez -= Lz * rintf(ez / Lz);
+          const float rs = (rif + r.w) * 1.000001f + margin;
+          if (!(ex * ex + ey * ey + ez * ez < rs * rs)) continue;
+          cand_scan(d, t, t + 1, (uint32_t)s, xi, ri, g, cnt, over);
         }
+      } else {
+        cand_scan(d, t0, t1, (uint32_t)s, xi, ri, g, cnt, over);
       }
-    }
+    });
     d.cand_count[s] = cnt < d.K ? cnt : d.K;
   }
   count_add(d.counters, kCntOverflow, over);
@@ -363,8 +524,11 @@ __global__ void __launch_bounds__(kBlock) k_candidates(DevScene d, const uint32_
 }
 
 void launch_candidates(const DevScene& d, const uint32_t* keys_s, hipStream_t s) {
-  hipLaunchKernelGGL(k_candidates, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d,
-                     keys_s);
+  const dim3 grid((unsigned)((d.n + d.block - 1) / d.block)), block(d.block);
+  if (d.cand_staged)
+    hipLaunchKernelGGL(k_candidates_staged, grid, block, 0, s, d, keys_s);
+  else
+    hipLaunchKernelGGL(k_candidates_direct, grid, block, 0, s, d, keys_s);
 }
 
 // flat entry list in sorted positions: entry e = entry_off[s] + k holds

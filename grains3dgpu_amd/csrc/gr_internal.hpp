@@ -8,6 +8,7 @@
 //   broad    key[N] (by particle), key/idx ping-pong [2][N], rank[N],
 //            cell_off[C+1] (first sorted position of each cell)      u32
 //   sorted   bp[N] = {x, y, z, prefilter radius}   32 B, cell order
+//            bp32[N] = bp in FP32, relative to the sim's domain minimum, 16 B
 //            pk[N] = PoseRec {pos, quat, shape, sim}  64 B, cell order
 //            (rebuilt every step from the planes; neighbours of a cell are
 //            contiguous, so the gathers of a wave share cache lines)
@@ -113,6 +114,7 @@ struct DevScene {
   const uint32_t* sidx;         // particle at each sorted position (this step)
   uint32_t* cell_off;           // CSR over cells: residents of c are [off[c], off[c+1])
   double4* bp;                  // {x, y, z, prefilter radius} by sorted position
+  float4* bp32;                 // bp in FP32, positions relative to the sim's domain minimum
   PoseRec* pk;                  // pose by sorted position
   uint32_t ncells;
   uint32_t *hist;               // 256 * n_tiles (+1)
@@ -131,6 +133,7 @@ struct DevScene {
   uint32_t* defer_off;          // exclusive scan of defer_count (+ total)
   uint32_t defer_cap;
   int32_t split;                // 1: screening pass + pass over the queued entries
+  int32_t cand_staged;          // 1: candidate search stages neighbour windows in LDS
   // per particle outputs
   int32_t* contact_count;
   double *last_force, *last_torque;

@@ -53,6 +53,7 @@ def lib():
         L.grains_set_profiling.argtypes = [vp, C.c_int]
         L.grains_set_launch.argtypes = [vp, C.c_int, C.c_int]
         L.grains_set_narrow_passes.argtypes = [vp, C.c_int]
+        L.grains_set_candidate_staging.argtypes = [vp, C.c_int]
         L.grains_adjoint_record.argtypes = [vp, C.c_uint32]
         L.grains_adjoint.argtypes = [vp, vp, vp, vp]
         L.grains_get_phase_times.argtypes = [vp, vp, C.c_int]
@@ -74,13 +75,14 @@ class Engine:
     """One grains_ctx_t.  precision: _abi.FP64 | _abi.FP32_MIXED."""
 
     def __init__(self, dt=1e-6, precision=A.FP64, flags=0, gjk_rel_tol=0.0, max_neighbors=0,
-                 block_size=0, narrow_blocks_per_cu=0, narrow_passes=0):
+                 block_size=0, narrow_blocks_per_cu=0, narrow_passes=0, candidate_staging=0):
         self.L = lib()
         cfg = A.Config()
         cfg.dt, cfg.precision, cfg.flags, cfg.gjk_rel_tol = dt, precision, flags, gjk_rel_tol
         cfg.max_neighbors, cfg.block_size = max_neighbors, block_size
         cfg.narrow_blocks_per_cu = narrow_blocks_per_cu
         cfg.narrow_passes = narrow_passes
+        cfg.candidate_staging = candidate_staging
         h = vp()
         rc = self.L.grains_create(C.byref(cfg), C.byref(h))
         if rc != 0:
@@ -220,6 +222,10 @@ class Engine:
     def set_narrow_passes(self, passes):
         """1: one complete narrow pass; 2: screening pass + pass over the queued entries."""
         self._chk(self.L.grains_set_narrow_passes(self.h, passes))
+
+    def set_candidate_staging(self, mode):
+        """0: automatic; 1: neighbours read from global memory; 2: staged in LDS."""
+        self._chk(self.L.grains_set_candidate_staging(self.h, mode))
 
     def set_profiling(self, on=True):
         self._chk(self.L.grains_set_profiling(self.h, 1 if on else 0))

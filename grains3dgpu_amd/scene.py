@@ -456,13 +456,20 @@ def config3(n_sims=512, n_range=(500, 20_000), seed=3, phi=0.3, jitter=0.05, cru
 
 
 def config4(n=8192, phi=0.35, seed=4, kn=1e5, en=0.8, mu=0.3, vel=0.1, crust_frac=0.05, max_rounds=60,
-            precision=None, log=None, resolve=True):
+            precision=None, log=None, resolve=True, a_rng=(2.5e-3, 4e-3), jitter=0.04, layout="layers"):
     """Differentiable calibration: n particles, half spheres (material 0, r ~
-    U[0.4, 0.6] mm), half ellipsoids (material 1, superquadrics n1 = n2 = 2,
-    a ~ U[0.4, 0.6] mm, b and c = a U[0.5, 1]: axis ratios up to 2:1), rho =
-    2500, on a jittered cubic lattice at solid fraction phi in a walled box,
-    random orientations, Gaussian velocities (sigma `vel` m/s) so that
-    collisions start at once; gravity -9.81 z, dt = 1e-6.  Contact tables:
+    U(a_rng)), half ellipsoids (material 1, superquadrics n1 = n2 = 2,
+    a ~ U(a_rng), b and c = a U[0.5, 1]: axis ratios up to 2:1), rho =
+    2500, placed at solid fraction phi (layout "layers": size-sorted lattice
+    layers, _size_layers, which reach phi with the 1.6:1 size spread; layout
+    "lattice": one cubic lattice at the mean volume's spacing, whose largest
+    grains overlap and are shrunk below), jittered by +-`jitter` of the
+    spacing, in a walled box, random orientations, Gaussian
+    velocities (sigma `vel` m/s); gravity -9.81 z, dt = 1e-6.  The sizes
+    keep the explicit step stable up to kn = 2e5: the lightest pair (two
+    ellipsoids of a = 2.5 mm, b = c = a/2) has sqrt(kn / m*) dt = 0.1 there
+    (grains of 0.5 mm would have 0.4 to 1.1, and explicit Euler gains energy
+    in every contact).  Contact tables:
     one block of the three material pairs (0-0, 0-1, 1-1; walls are material
     0) with the same kn, en, mu and the tangential damping matched to the
     normal one at these values (kept fixed: the calibrated parameters are kn,
@@ -471,7 +478,7 @@ def config4(n=8192, phi=0.35, seed=4, kn=1e5, en=0.8, mu=0.3, vel=0.1, crust_fra
     gets a new orientation, from round 4 its size shrunk by 0.9."""
     rng = np.random.default_rng(seed)
     kind = np.where(np.arange(n) % 2 == 0, A.SPHERE, A.SUPERQUADRIC).astype(np.int32)
-    a = rng.uniform(0.4e-3, 0.6e-3, n)
+    a = rng.uniform(a_rng[0], a_rng[1], n)
     dims = np.zeros((n, 5))
     dims[:, 0] = a
     sq = kind == A.SUPERQUADRIC
@@ -479,14 +486,23 @@ def config4(n=8192, phi=0.35, seed=4, kn=1e5, en=0.8, mu=0.3, vel=0.1, crust_fra
     dims[sq, 2] = a[sq] * rng.uniform(0.5, 1.0, sq.sum())
     dims[sq, 3:] = 2.0
     vol = np.where(sq, 4.0 / 3.0 * np.pi * dims[:, 0] * dims[:, 1] * dims[:, 2], 4.0 / 3.0 * np.pi * a ** 3)
-    sp = (float(vol.mean()) / phi) ** (1.0 / 3.0)
-    side = int(math.ceil(n ** (1.0 / 3.0)))
-    idx = np.arange(n)
-    cz, rem = np.divmod(idx, side * side)
-    cy, cx = np.divmod(rem, side)
-    margin = 0.7e-3
-    L = side * sp + 2 * margin
-    pos = margin + (np.stack([cx, cy, cz], 1) + 0.5) * sp + rng.uniform(-0.05, 0.05, (n, 3)) * sp
+    if layout == "layers":
+        pos, Lxy, ztop, pad = _size_layers(rng, vol, a, a, phi, jitter)
+        pad += 0.05 * a_rng[1]
+        lo = np.array([-pad, -pad, -pad])
+        hi = np.array([Lxy + pad, Lxy + pad, ztop + pad])
+        bed_vol = Lxy * Lxy * ztop
+    else:
+        sp = (float(vol.mean()) / phi) ** (1.0 / 3.0)
+        side = int(math.ceil(n ** (1.0 / 3.0)))
+        idx = np.arange(n)
+        cz, rem = np.divmod(idx, side * side)
+        cy, cx = np.divmod(rem, side)
+        margin = max(0.7e-3, 1.05 * a_rng[1])  # no particle reaches a wall
+        L = side * sp + 2 * margin
+        pos = margin + (np.stack([cx, cy, cz], 1) + 0.5) * sp + rng.uniform(-jitter, jitter, (n, 3)) * sp
+        lo, hi = np.zeros(3), np.full(3, L)
+        bed_vol = (L - 2 * margin) ** 3
     quat = random_quaternions(rng, n)
     rho = 2500.0
     m0 = rho * float(vol.mean())
@@ -498,9 +514,9 @@ def config4(n=8192, phi=0.35, seed=4, kn=1e5, en=0.8, mu=0.3, vel=0.1, crust_fra
                        dims=tuple(dims[i, :1] if kind[i] == A.SPHERE else dims[i]),
                        crust=crust_frac * float(dims[i, :1 if kind[i] == A.SPHERE else 3].min()), density=rho)
                   for i in range(n)]
-        ws, otr = box_walls([0, 0, 0], [L, L, L])
+        ws, otr = box_walls(lo, hi)
         return Scene(shapes=shapes + ws, tables=tables, n_materials=2,
-                     sims=[dict(domain_min=(0, 0, 0), domain_max=(L, L, L), gravity=(0, 0, -9.81))],
+                     sims=[dict(domain_min=tuple(lo), domain_max=tuple(hi), gravity=(0, 0, -9.81))],
                      particle_shape=np.arange(n, dtype=np.uint32), pos=pos.copy(), quat=quat.copy(),
                      vel=np.random.default_rng(seed + 1).normal(0.0, vel, (n, 3)), omega=np.zeros((n, 3)),
                      obstacle_shape=np.arange(n, n + len(ws), dtype=np.uint32), obstacle_tr=otr, dt=1e-6,
@@ -527,8 +543,43 @@ def config4(n=8192, phi=0.35, seed=4, kn=1e5, en=0.8, mu=0.3, vel=0.1, crust_fra
             raise RuntimeError("config4: overlaps remain")
     sc = make()
     vol = np.where(sq, 4.0 / 3.0 * np.pi * dims[:, 0] * dims[:, 1] * dims[:, 2], 4.0 / 3.0 * np.pi * dims[:, 0] ** 3)
-    sc.notes = dict(phi=float(vol.sum() / (L - 2 * margin) ** 3), rounds=rounds, steps=2000)
+    sc.notes = dict(phi=float(vol.sum() / bed_vol), rounds=rounds, steps=2000)
     return sc
+
+
+def _size_layers(rng, vol, rc, size, phi, jitter):
+    """Positions of particles sorted by `size`, largest first, in horizontal
+    square-lattice layers filling a column of side Lxy = (sum vol / phi)^(1/3)
+    from z = 0 up; each layer's spacing comes from its own mean volume and
+    phi (rows rounded to fit Lxy), each position jittered by +-`jitter` of the
+    layer spacing.  Returns (positions, Lxy, top of the last layer, pad): pad
+    is how far a particle (circumscribed radius rc) can reach beyond its
+    lattice cell, the margin the walls need."""
+    n = len(vol)
+    Lxy = (vol.sum() / phi) ** (1.0 / 3.0)
+    order = np.argsort(-size, kind="stable")
+    pos = np.zeros((n, 3))
+    z, i, pad = 0.0, 0, 0.0
+    while i < n:
+        s = (vol[order[i]] / phi) ** (1.0 / 3.0)
+        for _ in range(3):  # the layer's spacing from its own mean volume
+            row = max(1, int(round(Lxy / s)))
+            m = min(row * row, n - i)
+            s = (vol[order[i:i + m]].mean() / phi) ** (1.0 / 3.0)
+        row = max(1, int(round(Lxy / s)))
+        m = min(row * row, n - i)
+        sp = Lxy / row
+        k = np.arange(m)
+        cy, cx = np.divmod(k, row)
+        idx = order[i:i + m]
+        pos[idx, 0] = (cx + 0.5) * sp
+        pos[idx, 1] = (cy + 0.5) * sp
+        pos[idx, 2] = z + 0.5 * s
+        pos[idx] += rng.uniform(-jitter, jitter, (m, 3)) * s
+        pad = max(pad, float((rc[idx] - 0.45 * min(s, sp)).max()))
+        z += s
+        i += m
+    return pos, Lxy, z, pad
 
 
 # ---------------------------------------------------------------------------
@@ -580,29 +631,7 @@ def config2(n=250_000, phi=0.45, seed=2, r_rng=(2e-3, 6e-3), shares=(0.5, 0.3, 0
         return v, rc
 
     vol, rc = vol_rc(r)
-    Lxy = (vol.sum() / phi) ** (1.0 / 3.0)
-    order = np.argsort(-r, kind="stable")
-    pos = np.zeros((n, 3))
-    z, i, pad = 0.0, 0, 0.0
-    while i < n:
-        s = (vol[order[i]] / phi) ** (1.0 / 3.0)
-        for _ in range(3):  # the layer's spacing from its own mean volume
-            row = max(1, int(round(Lxy / s)))
-            m = min(row * row, n - i)
-            s = (vol[order[i:i + m]].mean() / phi) ** (1.0 / 3.0)
-        row = max(1, int(round(Lxy / s)))
-        m = min(row * row, n - i)
-        sp = Lxy / row
-        k = np.arange(m)
-        cy, cx = np.divmod(k, row)
-        idx = order[i:i + m]
-        pos[idx, 0] = (cx + 0.5) * sp
-        pos[idx, 1] = (cy + 0.5) * sp
-        pos[idx, 2] = z + 0.5 * s
-        pos[idx] += rng.uniform(-jitter, jitter, (m, 3)) * s
-        pad = max(pad, float((rc[idx] - 0.45 * min(s, sp)).max()))
-        z += s
-        i += m
+    pos, Lxy, z, pad = _size_layers(rng, vol, rc, r, phi, jitter)
     lo = np.array([-pad, -pad, -pad])
     hi = np.array([Lxy + pad, Lxy + pad, z + pad])
     box = kind == A.BOX

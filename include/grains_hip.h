@@ -128,7 +128,13 @@ typedef struct grains_config_t {
                                support pair) and a complete pass over the
                                undecided entries, compacted; 1 = one complete
                                pass (0: 2).  Results do not depend on it.   */
-  int32_t reserved[2];
+  int32_t candidate_staging; /* broad phase: 1 = every lane reads its
+                               neighbours' records from global memory; 2 = a
+                               block first stages its neighbour windows in
+                               LDS (pays off when cells hold many particles);
+                               0 = 2 when particles outnumber cells 8 to 1,
+                               else 1.  Results do not depend on it.        */
+  int32_t reserved[1];
 } grains_config_t;
 
 /* Counters of the last grains_step call (summed over its steps). */
@@ -202,6 +208,11 @@ int grains_set_launch(grains_ctx_t* ctx, int block_size, int narrow_blocks_per_c
  * changeable between steps; used by the autotuner.  No counterpart in the
  * reference (one pass, one lane per pair, ComponentManagerGPU_Kernels.cpp:141-226). */
 int grains_set_narrow_passes(grains_ctx_t* ctx, int passes);
+/* Candidate search mode (0, 1 or 2, see grains_config_t.candidate_staging),
+ * changeable between steps; used by the autotuner.  No counterpart in the
+ * reference (its particle loop reads every neighbour through the cell lists,
+ * ComponentManagerCPU.cpp:364-421). */
+int grains_set_candidate_staging(grains_ctx_t* ctx, int mode);
 
 /* Time stepping ------------------------------------------------------------ */
 /* n_steps iterations of the GrainsCPU::simulate loop body
