@@ -61,6 +61,11 @@ B_NP_PARTICLE = 64
 B_NP_SHAPE = 96
 B_NP_ENTRY = 12
 B_NP_CONTACT = 24
+# Compulsory HBM bytes of one candidate-search launch: each particle's FP64
+# record (32) and FP32 record (16), key, sim and candidate count (12) once;
+# each candidate written (4)
+B_CAND_PARTICLE = 60
+B_CAND_ENTRY = 4
 # Compulsory HBM bytes per particle-step of the FP32-mixed step at 3 radix
 # passes (state read 84 + write 80, key+index 8, sort 16 x 3, sorted key 4)
 B_PS_FP32 = 224
@@ -181,7 +186,21 @@ def shape_bytes(sc):
     return len(used) * B_NP_SHAPE + 12 * nv
 
 
-def narrow_roofline(sc, r, K, traffic_file=None):
+def measured_traffic(traffic_file):
+    """HBM bytes per launch by kernel, from the FETCH_SIZE and WRITE_SIZE
+    counter runs summarised in profiles/ (lines "hbm_bytes_per_launch
+    <kernel> = N"); empty until measured."""
+    out = {}
+    tf = os.path.join(ROOT, "profiles", traffic_file) if traffic_file else None
+    if tf and os.path.exists(tf):
+        for line in open(tf):
+            if line.strip().startswith("hbm_bytes_per_launch"):
+                key, val = line.split("=")
+                out[key.split()[1]] = float(val)
+    return out
+
+
+def narrow_roofline(sc, r, K, traffic=None):
     """Roofline of the screening narrow pass (k_narrow<float,true,2>, one
     launch per step): compulsory bytes per launch over its HIP-event time."""
     st = r["stats"]
@@ -189,21 +208,35 @@ def narrow_roofline(sc, r, K, traffic_file=None):
     E, C = st["candidate_pairs"] / K, st["contacts"] / K
     nbytes = sc.n * B_NP_PARTICLE + shape_bytes(sc) + E * B_NP_ENTRY + C * B_NP_CONTACT
     achieved_gbs = nbytes / (nar_ms * 1e-3) / 1e9
-    # HBM bytes per launch of the same kernel, from the FETCH_SIZE and
-    # WRITE_SIZE counter runs summarised in profiles/ (a line
-    # "hbm_bytes_per_launch = N"); null until measured
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", traffic_file) if traffic_file else None
-    if tf and os.path.exists(tf):
-        for line in open(tf):
-            if line.strip().startswith("hbm_bytes_per_launch"):
-                traffic = float(line.split("=")[1])
-    return dict(bound="hbm", kernel="k_narrow<float,true,2>", achieved=achieved_gbs, peak=PEAK_HBM_GBS,
-                unit="GB/s", frac=achieved_gbs / PEAK_HBM_GBS, traffic=traffic,
+    kernel = "k_narrow<float,true,2>"
+    return dict(bound="hbm", kernel=kernel, achieved=achieved_gbs, peak=PEAK_HBM_GBS,
+                unit="GB/s", frac=achieved_gbs / PEAK_HBM_GBS, traffic=(traffic or {}).get(kernel),
                 bytes_per_launch=nbytes, avg_launch_ms=nar_ms,
                 note="screening narrow pass (every candidate entry); compulsory bytes: 64 B pose+ids per "
                      "particle, 96 B per shape record (+12 B per polyhedron vertex), 12 B per candidate "
                      "entry, 24 B per contact; the launch is gather-latency bound")
+
+
+def candidates_roofline(sc, r, K, staged, traffic=None):
+    """Roofline of the candidate search (one launch per step, the step's
+    largest kernel): compulsory bytes per launch over its HIP-event time.
+    Compulsory: each particle's FP64 and FP32 records (32 + 16 B), its key,
+    sim and count (12 B) once, each cell's offset (4 B), each candidate
+    written (4 B).  Every particle checks ~200 residents of its 27 cells:
+    those 16-byte reads are served by the caches (L1 / L2 or LDS), not HBM,
+    and bound the kernel."""
+    st = r["stats"]
+    ms = r["phases"]["candidates"]
+    E = st["candidate_pairs"] / K
+    nbytes = sc.n * B_CAND_PARTICLE + E * B_CAND_ENTRY
+    achieved_gbs = nbytes / (ms * 1e-3) / 1e9
+    kernel = "k_candidates_staged" if staged else "k_candidates_direct"
+    return dict(bound="hbm", kernel=kernel, achieved=achieved_gbs, peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=achieved_gbs / PEAK_HBM_GBS, traffic=(traffic or {}).get(kernel),
+                bytes_per_launch=nbytes, avg_launch_ms=ms,
+                note="candidate search; compulsory bytes: 60 B per particle (FP64 + FP32 records, key, sim, "
+                     "count), 4 B per candidate (cell offsets, 4 B per cell, not counted); bound by the "
+                     "cache traffic of ~200 resident checks per particle, not by HBM")
 
 
 def valu_roofline(r, K):
@@ -265,7 +298,9 @@ def bench_c1(args):
         f"rounds = {sc.notes['rounds']}, redraws = {sc.notes['redraws']}")
 
     log(f"FP32-mixed: {W} warm-up + profiled + {K} timed steps")
-    r32 = run_engine(sc, A.FP32_MIXED, W, K)
+    # a few residents per cell: the direct candidate search (what the
+    # automatic choice takes here)
+    r32 = run_engine(sc, A.FP32_MIXED, W, K, candidate_staging=1)
     ps32 = sc.n * K / r32["wall_s"]
     log(f"  {r32['wall_s'] * 1e3 / K:.3f} ms/step, {ps32:.3e} particle-steps/s")
     r64 = None
@@ -282,7 +317,8 @@ def bench_c1(args):
         p64 = run_engine(sc, A.FP64, W, K64, profile=False)
         err = dict(steps=W + K64, **pos_error(p32, p64, sc.pos))
 
-    roofline = narrow_roofline(sc, r32, K, "narrow32_c1_1M_traffic.txt")
+    traffic = measured_traffic("step_c1_1M_traffic.txt")
+    roofline = candidates_roofline(sc, r32, K, False, traffic)
     step_ms = r32["wall_s"] * 1e3 / K
     roofline_step = dict(bound="hbm", bytes_per_particle_step=B_PS_FP32,
                          achieved=B_PS_FP32 * ps32 / 1e9, peak=PEAK_HBM_GBS, unit="GB/s",
@@ -343,6 +379,7 @@ def bench_c1(args):
         phase_ms_per_step=r32["phases"],
         gjk_microbench={"fp64": mb64, "fp32": mb32},
         roofline=roofline,
+        roofline_narrow=narrow_roofline(sc, r32, K, traffic),
         roofline_valu=valu_roofline(r32, K),
         roofline_step_hbm=roofline_step,
         cpu_baseline=cpu)
@@ -364,7 +401,9 @@ def bench_c2(args):
     log(f"  done in {time.perf_counter() - t0:.1f} s: phi = {sc.notes['phi']:.4f}, rounds = {sc.notes['rounds']}")
     tol = GJK_TOL
     log(f"FP32-mixed: {W} warm-up + profiled + {K} timed steps")
-    r32 = run_engine(sc, A.FP32_MIXED, W, K, max_neighbors=96)
+    # ~145 residents per cell: the LDS-staged candidate search (what the
+    # automatic choice takes here)
+    r32 = run_engine(sc, A.FP32_MIXED, W, K, max_neighbors=96, candidate_staging=2)
     ps32 = sc.n * K / r32["wall_s"]
     log(f"  {r32['wall_s'] * 1e3 / K:.3f} ms/step, {ps32:.3e} particle-steps/s")
     r64 = None
@@ -394,7 +433,8 @@ def bench_c2(args):
         gjk_pair_tests_per_s=st["gjk_calls"] / r32["wall_s"],
         gjk_iterations_per_test=st["gjk_iterations"] / max(st["gjk_calls"], 1),
         step_stats=step_stats(st), phase_ms_per_step=r32["phases"],
-        roofline=narrow_roofline(sc, r32, K), roofline_valu=valu_roofline(r32, K), cpu_baseline=cpu,
+        roofline=candidates_roofline(sc, r32, K, True), roofline_narrow=narrow_roofline(sc, r32, K),
+        roofline_valu=valu_roofline(r32, K), cpu_baseline=cpu,
         speedup_vs_cpu=None if cpu is None else ps32 / cpu["value"])
 
 
@@ -441,7 +481,8 @@ def bench_c3(args):
         default_launch_particle_steps_per_s=sc.n * min(K, 200) / d32["wall_s"],
         step_stats=step_stats(r32["stats"]), phase_ms_per_step=r32["phases"],
         gjk_pair_tests_per_s=r32["stats"]["gjk_calls"] / r32["wall_s"],
-        roofline=narrow_roofline(sc, r32, K), roofline_valu=valu_roofline(r32, K), cpu_baseline=cpu,
+        roofline=candidates_roofline(sc, r32, K, tuned["candidate_staging"] == 2),
+        roofline_narrow=narrow_roofline(sc, r32, K), roofline_valu=valu_roofline(r32, K), cpu_baseline=cpu,
         speedup_vs_cpu=None if cpu is None else ps32 / cpu["value"])
 
 
