@@ -249,8 +249,14 @@ void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx
 // empty ranges.  (Filling the empty cells from the thread of the particle
 // before them serialises the empty layers at the floor and the ceiling of a
 // bed on one thread each.)
-__global__ void k_cell_off(const uint32_t* __restrict__ ks, uint64_t n, uint64_t ncells,
-                           uint32_t* __restrict__ off) {
+// The same thread also writes the cell's bounding sphere, cell_sph[c] =
+// {centre, R} in the FP32 frame of bp32: every resident's prefilter sphere
+// lies inside it (R = max |x_j - centre| + r_j over the residents, inflated
+// by 1e-6 of the magnitudes involved, several times the FP32 rounding of
+// the records and of this computation); empty cells get R = -inf.  The
+// candidate search skips a neighbour cell whose sphere its own cannot reach.
+__global__ void k_cell_off(const uint32_t* __restrict__ ks, const float4* __restrict__ bp32, uint64_t n,
+                           uint64_t ncells, uint32_t* __restrict__ off, float4* __restrict__ sph) {
   const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c > ncells) return;
   uint64_t lo = 0, hi = n;
@@ -262,6 +268,27 @@ __global__ void k_cell_off(const uint32_t* __restrict__ ks, uint64_t n, uint64_t
       hi = mid;
   }
   off[c] = (uint32_t)lo;
+  if (c == ncells || !sph) return;
+  uint64_t end = lo;
+  float mn[3] = {3e38f, 3e38f, 3e38f}, mx[3] = {-3e38f, -3e38f, -3e38f};
+  while (end < n && (uint64_t)ks[end] == c) {
+    const float4 r = bp32[end++];
+    mn[0] = fminf(mn[0], r.x), mn[1] = fminf(mn[1], r.y), mn[2] = fminf(mn[2], r.z);
+    mx[0] = fmaxf(mx[0], r.x), mx[1] = fmaxf(mx[1], r.y), mx[2] = fmaxf(mx[2], r.z);
+  }
+  if (end == lo) {
+    sph[c] = make_float4(0.f, 0.f, 0.f, -3e38f);
+    return;
+  }
+  const float cx = 0.5f * (mn[0] + mx[0]), cy = 0.5f * (mn[1] + mx[1]), cz = 0.5f * (mn[2] + mx[2]);
+  float R = 0.f;
+  for (uint64_t t = lo; t < end; ++t) {
+    const float4 r = bp32[t];
+    const float ex = r.x - cx, ey = r.y - cy, ez = r.z - cz;
+    R = fmaxf(R, sqrtf(ex * ex + ey * ey + ez * ez) + r.w);
+  }
+  const float m = fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz))) + R;
+  sph[c] = make_float4(cx, cy, cz, R * 1.000001f + 1e-6f * m);
 }
 
 // One thread per sorted position s: the inverse permutation and the
@@ -294,11 +321,12 @@ __global__ void k_cells(DevScene d, const uint32_t* __restrict__ ks, const uint3
 
 void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s) {
   const uint64_t offs = (uint64_t)d.ncells + 1;
-  hipLaunchKernelGGL(k_cell_off, dim3((unsigned)((offs + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, keys_s, d.n,
-                     (uint64_t)d.ncells, d.cell_off);
-  if (d.n == 0) return;
-  hipLaunchKernelGGL(k_cells, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d, keys_s,
-                     idx_s);
+  if (d.n)  // the records first: the cell spheres are built from bp32
+    hipLaunchKernelGGL(k_cells, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d, keys_s,
+                       idx_s);
+  // the cell spheres serve the direct candidate search only
+  hipLaunchKernelGGL(k_cell_off, dim3((unsigned)((offs + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, keys_s, d.bp32,
+                     d.n, (uint64_t)d.ncells, d.cell_off, d.cand_staged ? nullptr : d.cell_sph);
 }
 
 // ---- candidates -------------------------------------------------------------------
@@ -360,14 +388,15 @@ __device__ __forceinline__ void for_each_neighbor(const SimGrid& g, uint32_t key
 }
 
 // Every lane reads its neighbours' records from global memory (lanes of a
-// cell read the same lines, which the caches serve).  The resident checks --
-// ~200 per particle, nearly all rejected -- read the 16-byte FP32 records
-// bp32 (positions relative to the sim's domain minimum) with a conservative
-// test; every pair it passes is decided by the FP64 test of cand_scan, so
-// the candidate lists are those of the FP64 prefilter bit for bit.  The
-// relative coordinates carry at most 2^-24 of their magnitude, the
-// difference and the periodic shift as much again of theirs; the margin is
-// 1e-6 of those magnitudes, several times the bound.
+// cell read the same lines, which the caches serve).  A neighbour cell whose
+// bounding sphere (cell_sph) is out of reach is skipped whole; the resident
+// checks -- ~90 per particle on the superquadric bed, nearly all rejected --
+// read the 16-byte FP32 records bp32 (positions relative to the sim's domain
+// minimum) with a conservative test; every pair it passes is decided by the
+// FP64 test of cand_scan, so the candidate lists are those of the FP64
+// prefilter bit for bit.  The relative coordinates carry at most 2^-24 of
+// their magnitude, the difference and the periodic shift as much again of
+// theirs; the margin is 1e-6 of those magnitudes, several times the bound.
 __global__ void __launch_bounds__(kBlock) k_candidates_direct(DevScene d, const uint32_t* __restrict__ ks) {
   const uint64_t n = d.n;
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -384,19 +413,25 @@ __global__ void __launch_bounds__(kBlock) k_candidates_direct(DevScene d, const 
     const float iLx = 1.f / Lx, iLy = 1.f / Ly, iLz = 1.f / Lz;
     const float base = fmaxf(fabsf(fi.x), fmaxf(fabsf(fi.y), fabsf(fi.z))) +
                        3.f * (g.periodic ? fmaxf(Lx, fmaxf(Ly, Lz)) : 0.f);
+    // conservative FP32 reach test of a sphere q (centre, radius) from fi
+    auto reach = [&](const float4& q) {
+      float ex = q.x - fi.x, ey = q.y - fi.y, ez = q.z - fi.z;
+      if (px) ex -= Lx * rintf(ex * iLx);
+      if (py) ey -= Ly * rintf(ey * iLy);
+      if (pz) ez -= Lz * rintf(ez * iLz);
+      const float mq = fmaxf(fabsf(q.x), fmaxf(fabsf(q.y), fabsf(q.z)));
+      const float rs = (fi.w + q.w) * 1.000001f + 1e-6f * (base + mq);
+      return ex * ex + ey * ey + ez * ez < rs * rs;
+    };
+    // (loading a row's three spheres and offsets together was measured no
+    // faster, at 78 instead of 64 VGPRs)
     for_each_neighbor(g, ks[s], [&](int, uint32_t c) {
+      const float4 q = d.cell_sph[c];
+      if (!(q.w >= 0.f && reach(q))) return;  // empty, or out of reach
       const uint32_t t1 = d.cell_off[c + 1];
       for (uint32_t t = d.cell_off[c]; t < t1; ++t) {
         if (t == (uint32_t)s) continue;
-        const float4 r = d.bp32[t];
-        float ex = r.x - fi.x, ey = r.y - fi.y, ez = r.z - fi.z;
-        if (px) ex -= Lx * rintf(ex * iLx);
-        if (py) ey -= Ly * rintf(ey * iLy);
-        if (pz) ez -= Lz * rintf(ez * iLz);
-        const float mj = fmaxf(fabsf(r.x), fmaxf(fabsf(r.y), fabsf(r.z)));
-        const float rs = (fi.w + r.w) * 1.000001f + 1e-6f * (base + mj);
-        if (!(ex * ex + ey * ey + ez * ez < rs * rs)) continue;
-        cand_scan(d, t, t + 1, (uint32_t)s, xi, bi.w, g, cnt, over);
+        if (reach(d.bp32[t])) cand_scan(d, t, t + 1, (uint32_t)s, xi, bi.w, g, cnt, over);
       }
     });
     d.cand_count[s] = cnt < d.K ? cnt : d.K;

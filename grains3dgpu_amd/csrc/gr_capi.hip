@@ -700,6 +700,7 @@ static int finalize(grains_ctx* ctx) {
       (rc = dalloc(ctx, &d.cell_off, ((size_t)d.ncells + 1) * 4, L)) ||
       (rc = dalloc(ctx, &d.bp, n * sizeof(double4), L)) || (rc = dalloc(ctx, &d.pk, n * sizeof(PoseRec), L)) ||
       (rc = dalloc(ctx, &d.bp32, n * sizeof(float4), L)) ||
+      (rc = dalloc(ctx, &d.cell_sph, ((size_t)d.ncells + 1) * sizeof(float4), L)) ||
       (rc = dalloc(ctx, &d.hist, (256 * ntiles + 1) * 4, L)) || (rc = dalloc(ctx, &d.scan_blocks, nblk * 4, L)) ||
       (rc = dalloc(ctx, &d.cand, ecap * 4, L)) || (rc = dalloc(ctx, &d.cand_count, n * 4, L)) ||
       (rc = dalloc(ctx, &d.entry_off, (n + 1) * 4, L)) || (rc = dalloc(ctx, &d.pair_i, ecap * 4, L)) ||

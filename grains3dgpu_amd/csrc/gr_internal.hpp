@@ -9,6 +9,7 @@
 //            cell_off[C+1] (first sorted position of each cell)      u32
 //   sorted   bp[N] = {x, y, z, prefilter radius}   32 B, cell order
 //            bp32[N] = bp in FP32, relative to the sim's domain minimum, 16 B
+//            cell_sph[C] = bounding sphere of each cell's residents, 16 B
 //            pk[N] = PoseRec {pos, quat, shape, sim}  64 B, cell order
 //            (rebuilt every step from the planes; neighbours of a cell are
 //            contiguous, so the gathers of a wave share cache lines)
@@ -115,6 +116,7 @@ struct DevScene {
   uint32_t* cell_off;           // CSR over cells: residents of c are [off[c], off[c+1])
   double4* bp;                  // {x, y, z, prefilter radius} by sorted position
   float4* bp32;                 // bp in FP32, positions relative to the sim's domain minimum
+  float4* cell_sph;             // per cell: bounding sphere of its residents' spheres (bp32 frame)
   PoseRec* pk;                  // pose by sorted position
   uint32_t ncells;
   uint32_t *hist;               // 256 * n_tiles (+1)
