@@ -2,7 +2,8 @@
 //
 //   k_hash        cell key of every particle (LinkedCell::computeLinearCellHash,
 //                 LinkedCell.cpp:117-149), FP64 divide + floor as the reference
-//   radix sort    stable LSD sort of (key, index), 8-bit digits: per-tile LDS
+//   radix sort    stable LSD sort of (key, index), digits of 8-11 bits (the
+//                 fewest passes for the key width): per-tile LDS
 //                 histogram, device-wide exclusive scan, stable scatter ranked
 //                 with wave ballots (replaces thrust::sort_by_key,
 //                 ComponentManagerGPU.cpp:282-285)
@@ -90,35 +91,45 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
 // ---- radix sort -----------------------------------------------------------------
 // Elements of tile b are b*kTile + k*kBlock + tid, k = 0..kItems-1: the item
 // slot k outermost, so a stable rank follows (k, wave, lane).
+// BITS-bit digits (8 .. kRadixMaxBits): D = 2^BITS counters per tile, each
+// thread owns digits tid, tid + kBlock, ...; hist is digit-major,
+// hist[d * ntiles + tile], so one exclusive scan gives every (digit, tile)
+// its first output position.
+template <int BITS>
 __global__ void __launch_bounds__(kBlock) k_radix_hist(uint64_t n, const uint32_t* __restrict__ keys,
                                                       int shift, uint32_t ntiles, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t cnt[256];
+  constexpr uint32_t D = 1u << BITS;
+  __shared__ uint32_t cnt[D];
   const uint32_t tid = threadIdx.x;
-  cnt[tid] = 0;
+  for (uint32_t dg = tid; dg < D; dg += kBlock) cnt[dg] = 0;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     uint64_t e = base + (uint64_t)k * kBlock + tid;
-    if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 255u], 1u);
+    if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & (D - 1u)], 1u);
   }
   __syncthreads();
-  hist[(uint64_t)tid * ntiles + blockIdx.x] = cnt[tid];
+  for (uint32_t dg = tid; dg < D; dg += kBlock) hist[(uint64_t)dg * ntiles + blockIdx.x] = cnt[dg];
 }
 
+template <int BITS>
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(uint64_t n, const uint32_t* __restrict__ kin,
                                                          const uint32_t* __restrict__ iin, int shift,
                                                          uint32_t ntiles, const uint32_t* __restrict__ offs,
                                                          uint32_t* __restrict__ kout, uint32_t* __restrict__ iout,
                                                          int identity_in) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[kBlock / 64][256];
+  constexpr uint32_t D = 1u << BITS;
+  __shared__ uint32_t run[D];
+  __shared__ uint32_t wcnt[kBlock / 64][D];
   const uint32_t tid = threadIdx.x;
   const int lane = __lane_id();
   const int wave = tid / 64;
-  run[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
+  for (uint32_t dg = tid; dg < D; dg += kBlock) {
+    run[dg] = offs[(uint64_t)dg * ntiles + blockIdx.x];
 #pragma unroll
-  for (int w = 0; w < kBlock / 64; ++w) wcnt[w][tid] = 0;
+    for (int w = 0; w < kBlock / 64; ++w) wcnt[w][dg] = 0;
+  }
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -127,11 +138,11 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(uint64_t n, const uint
     const bool valid = e < n;
     uint32_t key = valid ? kin[e] : 0u;
     uint32_t idx = valid ? (identity_in ? (uint32_t)e : iin[e]) : 0u;
-    uint32_t dig = (key >> shift) & 255u;
+    uint32_t dig = (key >> shift) & (D - 1u);
     // lanes of this wave holding the same digit
     uint64_t m = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < BITS; ++b) {
       uint64_t bb = __ballot((dig >> b) & 1u);
       m &= ((dig >> b) & 1u) ? bb : ~bb;
     }
@@ -139,15 +150,15 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(uint64_t n, const uint
     if (valid && rank == 0) wcnt[wave][dig] = __popcll(m);
     __syncthreads();
     // per digit: offsets of the waves in order, then advance the running count
-    {
-      uint32_t r = run[tid];
+    for (uint32_t dg = tid; dg < D; dg += kBlock) {
+      uint32_t r = run[dg];
 #pragma unroll
       for (int w = 0; w < kBlock / 64; ++w) {
-        uint32_t c = wcnt[w][tid];
-        wcnt[w][tid] = r;
+        uint32_t c = wcnt[w][dg];
+        wcnt[w][dg] = r;
         r += c;
       }
-      run[tid] = r;
+      run[dg] = r;
     }
     __syncthreads();
     if (valid) {
@@ -156,13 +167,15 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(uint64_t n, const uint
       iout[p] = idx;
     }
     __syncthreads();
+    for (uint32_t dg = tid; dg < D; dg += kBlock) {
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) wcnt[w][tid] = 0;
+      for (int w = 0; w < kBlock / 64; ++w) wcnt[w][dg] = 0;
+    }
     __syncthreads();
   }
 }
 
-// ---- device-wide exclusive scan (reduce, top, down-sweep) -------------------------
+// ---- device-wide exclusive scan (block sums, down-sweep) --------------------------
 __global__ void __launch_bounds__(kBlock) k_scan_reduce(uint64_t n, const uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ bsum) {
   __shared__ uint32_t lw[kBlock / 64];
@@ -176,25 +189,20 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(uint64_t n, const uint32
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// one block: exclusive scan of nb block sums in place, total in bsum[nb]
-__global__ void __launch_bounds__(1024) k_scan_top(uint32_t nb, uint32_t* __restrict__ bsum) {
-  __shared__ uint32_t lw[16];
-  uint32_t carry = 0;
-  for (uint32_t c = 0; c < nb; c += 1024) {
-    uint32_t i = c + threadIdx.x;
-    uint32_t v = i < nb ? bsum[i] : 0u;
-    uint32_t tot;
-    uint32_t ex = block_exclusive_scan(v, lw, tot);
-    if (i < nb) bsum[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) bsum[nb] = carry;
-}
-
+// Down-sweep: each block first sums the preceding blocks' sums itself (a few
+// hundred at most: cheaper than a separate launch scanning them), then scans
+// its tile.  In place (out == in) is safe: a block reads its tile before it
+// writes it, and only the block sums are read across blocks.
 __global__ void __launch_bounds__(kBlock) k_scan_down(uint64_t n, const uint32_t* __restrict__ in,
                                                      uint32_t* __restrict__ out,
                                                      const uint32_t* __restrict__ bsum, uint32_t nb) {
   __shared__ uint32_t lw[kBlock / 64];
+  __shared__ uint32_t pw[kBlock / 64];
+  uint32_t p = 0;
+  for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kBlock) p += bsum[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o);
+  if (__lane_id() == 0) pw[threadIdx.x / 64] = p;
   const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
   uint32_t v[kItems];
   uint32_t s = 0;
@@ -204,39 +212,54 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(uint64_t n, const uint32_t
     s += v[k];
   }
   uint32_t tot;
-  uint32_t ex = block_exclusive_scan(s, lw, tot) + bsum[blockIdx.x];
+  uint32_t ex = block_exclusive_scan(s, lw, tot);  // its barriers also publish pw
+  uint32_t prefix = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) prefix += pw[w];
+  ex += prefix;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     if (base + k < n) out[base + k] = ex;
     ex += v[k];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = prefix + tot;
 }
 
-// out has n+1 entries; block_sums needs ceil(n/kTile)+1 entries
+// out has n+1 entries; block_sums needs ceil(n/kTile) entries.  One tile: one
+// launch.
 void launch_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* block_sums, hipStream_t s) {
   uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
   if (nb == 0) nb = 1;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, n, in, block_sums);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, nb, block_sums);
+  if (nb > 1) hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, n, in, block_sums);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, n, in, out, block_sums, nb);
 }
 
-void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx_sorted, hipStream_t s) {
+template <int BITS>
+static void run_sort_passes(DevScene& d, int passes, const uint32_t*& kin, const uint32_t*& iin, hipStream_t s) {
   const uint64_t n = d.n;
   const uint32_t nt = (uint32_t)((n + kTile - 1) / kTile);
-  const uint32_t* kin = d.key;
-  const uint32_t* iin = nullptr;
   uint32_t* ko[2] = {d.keyA, d.keyB};
   uint32_t* io[2] = {d.idxA, d.idxB};
   for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(kBlock), 0, s, n, kin, shift, nt, d.hist);
-    launch_scan(d.hist, d.hist, (uint64_t)256 * nt, d.scan_blocks, s);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(kBlock), 0, s, n, kin, iin, shift, nt, d.hist,
+    const int shift = BITS * p;
+    hipLaunchKernelGGL(k_radix_hist<BITS>, dim3(nt), dim3(kBlock), 0, s, n, kin, shift, nt, d.hist);
+    launch_scan(d.hist, d.hist, (uint64_t)(1u << BITS) * nt, d.scan_blocks, s);
+    hipLaunchKernelGGL(k_radix_scatter<BITS>, dim3(nt), dim3(kBlock), 0, s, n, kin, iin, shift, nt, d.hist,
                        ko[p & 1], io[p & 1], p == 0 ? 1 : 0);
     kin = ko[p & 1];
     iin = io[p & 1];
+  }
+}
+
+void launch_sort(DevScene& d, int passes, int digit_bits, uint32_t** keys_sorted, uint32_t** idx_sorted,
+                 hipStream_t s) {
+  const uint32_t* kin = d.key;
+  const uint32_t* iin = nullptr;
+  switch (digit_bits) {
+    case 8: run_sort_passes<8>(d, passes, kin, iin, s); break;
+    case 9: run_sort_passes<9>(d, passes, kin, iin, s); break;
+    case 10: run_sort_passes<10>(d, passes, kin, iin, s); break;
+    default: run_sort_passes<11>(d, passes, kin, iin, s); break;
   }
   *keys_sorted = const_cast<uint32_t*>(kin);
   *idx_sorted = const_cast<uint32_t*>(iin);

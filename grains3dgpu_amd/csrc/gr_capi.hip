@@ -307,7 +307,8 @@ struct grains_ctx {
   DevScene d;
   std::vector<void*> allocs;       // layout-dependent buffers
   std::vector<void*> state_allocs; // particle state
-  int passes = 1;
+  int passes = 1;      // LSD radix passes of the cell keys
+  int digit_bits = 8;  // bits per pass
   uint32_t *keys_s = nullptr, *idx_s = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -636,7 +637,10 @@ static int finalize(grains_ctx* ctx) {
   d.ncells = (uint32_t)base;
   int bits = 1;
   while (bits < 32 && (1ull << bits) < base) ++bits;
-  ctx->passes = (bits + 7) / 8;
+  // fewest LSD passes of digits up to kRadixMaxBits wide, digits at least 8
+  // bits: an 18-bit key sorts in 2 passes of 9 bits instead of 3 of 8
+  ctx->passes = (bits + kRadixMaxBits - 1) / kRadixMaxBits;
+  ctx->digit_bits = std::max(8, (bits + ctx->passes - 1) / ctx->passes);
   // shape tables
   const size_t S = ctx->shapes.size();
   std::vector<ShapeNP<double>> np64(S);
@@ -674,8 +678,9 @@ static int finalize(grains_ctx* ctx) {
   // the narrow phase's launch may hold up to 64 blocks per CU of up to 256
   // threads (grains_set_launch); the screen's queue has one segment per block
   const uint64_t max_nb = (uint64_t)cu_count() * 64;
+  const uint64_t nhist = (uint64_t)kRadixMaxDigits * ntiles;  // histogram of the widest digit
   const uint64_t nblk =
-      std::max(std::max((n + kTile - 1) / kTile, (256 * ntiles + kTile - 1) / kTile), (max_nb + kTile) / kTile) + 2;
+      std::max(std::max((n + kTile - 1) / kTile, (nhist + kTile - 1) / kTile), (max_nb + kTile) / kTile) + 2;
   const size_t res_sz = ctx->cfg.precision == GRAINS_FP64 ? sizeof(PairRes<double>) : sizeof(PairRes<float>);
   int rc;
   ShapeNP<double>* p64;
@@ -701,7 +706,7 @@ static int finalize(grains_ctx* ctx) {
       (rc = dalloc(ctx, &d.bp, n * sizeof(double4), L)) || (rc = dalloc(ctx, &d.pk, n * sizeof(PoseRec), L)) ||
       (rc = dalloc(ctx, &d.bp32, n * sizeof(float4), L)) ||
       (rc = dalloc(ctx, &d.cell_sph, ((size_t)d.ncells + 1) * sizeof(float4), L)) ||
-      (rc = dalloc(ctx, &d.hist, (256 * ntiles + 1) * 4, L)) || (rc = dalloc(ctx, &d.scan_blocks, nblk * 4, L)) ||
+      (rc = dalloc(ctx, &d.hist, (nhist + 1) * 4, L)) || (rc = dalloc(ctx, &d.scan_blocks, nblk * 4, L)) ||
       (rc = dalloc(ctx, &d.cand, ecap * 4, L)) || (rc = dalloc(ctx, &d.cand_count, n * 4, L)) ||
       (rc = dalloc(ctx, &d.entry_off, (n + 1) * 4, L)) || (rc = dalloc(ctx, &d.pair_i, ecap * 4, L)) ||
       (rc = dalloc(ctx, &d.pair_j, ecap * 4, L)) || (rc = dalloc(ctx, &d.pair_flag, ecap * 4, L)) ||
@@ -879,7 +884,7 @@ static void enqueue_step(grains_ctx* ctx) {
   if (d.precision == GRAINS_FP32_MIXED) launch_fill_u32(d.refine_count, 1, 0u, s);
   launch_hash(d, s);
   mark(ctx);
-  launch_sort(d, ctx->passes, &ctx->keys_s, &ctx->idx_s, s);
+  launch_sort(d, ctx->passes, ctx->digit_bits, &ctx->keys_s, &ctx->idx_s, s);
   mark(ctx);
   d.sidx = ctx->idx_s;
   launch_cells(d, ctx->keys_s, ctx->idx_s, s);

@@ -30,6 +30,8 @@ namespace gr {
 constexpr int kBlock = 256;          // threads per block of streaming kernels
 constexpr int kItems = 8;            // items per thread in sort / scan tiles
 constexpr int kTile = kBlock * kItems;
+constexpr int kRadixMaxBits = 11;    // widest radix digit (LDS histograms of 2048 counters)
+constexpr int kRadixMaxDigits = 1 << kRadixMaxBits;
 constexpr double kPrefilterSlack = 1.e-10;
 
 // per-pair flag bits
@@ -169,8 +171,10 @@ struct AdjBufs {
 // gr_broad.hip
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 void launch_hash(const DevScene& d, hipStream_t s);
-// sorts (d.key, identity) into the returned buffers; passes = radix passes
-void launch_sort(DevScene& d, int passes, uint32_t** keys_sorted, uint32_t** idx_sorted, hipStream_t s);
+// sorts (d.key, identity) into the returned buffers: `passes` LSD passes of
+// `digit_bits` (8 .. kRadixMaxBits) bits each
+void launch_sort(DevScene& d, int passes, int digit_bits, uint32_t** keys_sorted, uint32_t** idx_sorted,
+                 hipStream_t s);
 // cell ranges, inverse permutation and the sorted pose records
 void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s);
 void launch_candidates(const DevScene& d, const uint32_t* keys_s, hipStream_t s);

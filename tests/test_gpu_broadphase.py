@@ -1,0 +1,82 @@
+"""The radix sort at every digit width it uses: keys of 6 to 25 bits (1 pass
+of 8 bits up to 3 passes of 9), on sparse sphere scenes whose domains hold
+2^bits cells.  The sorted order must be the stable argsort of the keys and
+the candidate lists those of a brute-force bounding-sphere search in the
+reference visit order (the 27-cell order of LinkedCell.cpp:41-70, residents
+ascending)."""
+import numpy as np
+import pytest
+
+from grains3dgpu_amd import _abi as A
+from grains3dgpu_amd import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def sparse_spheres(bits, n=4000, seed=0):
+    n = min(n, 2 ** bits // 4)  # at most one sphere per four cells
+    rng = np.random.default_rng(seed + bits)
+    r = 0.5e-3
+    h = 2 * r  # the cell size the library picks: 2 max circumscribed radius
+    side = np.ceil((2.0 ** bits) ** (1 / 3)) * h
+    # clusters, so that some spheres touch
+    centres = rng.uniform(2 * h, side - 2 * h, (n // 4, 3))
+    pos = np.repeat(centres, 4, axis=0) + rng.uniform(-0.9e-3, 0.9e-3, (n, 3))
+    pos = np.clip(pos, r, side - r)
+    return S.Scene(shapes=[dict(kind=A.SPHERE, dims=(r,), density=2500.0)],
+                   tables=np.array([S.hooke_row(1e5, 0.8, 2e4, 0.3)]), n_materials=1,
+                   sims=[dict(domain_min=(0, 0, 0), domain_max=(side, side, side), gravity=(0, 0, 0))],
+                   particle_shape=np.zeros(n, np.uint32), pos=pos, quat=np.tile([0, 0, 0, 1.0], (n, 1)),
+                   vel=np.zeros((n, 3)), omega=np.zeros((n, 3)), dt=1e-7)
+
+
+def brute_candidates(sc, keys, n3):
+    """(i, j) in the reference order: for each i, the 27 cells z, y, x outer
+    to inner, residents ascending, bounding spheres overlapping."""
+    n = sc.n
+    r = sc.shapes[0]["dims"][0]
+    order = np.argsort(keys, kind="stable")
+    ks = keys[order]
+    nx, ny, nz = n3
+    out = []
+    for i in range(n):
+        c = int(keys[i])
+        cx, cy, cz = c % nx, (c // nx) % ny, c // (nx * ny)
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    x, y, z = cx + dx, cy + dy, cz + dz
+                    if not (0 <= x < nx and 0 <= y < ny and 0 <= z < nz):
+                        continue
+                    cell = (z * ny + y) * nx + x
+                    lo, hi = np.searchsorted(ks, cell), np.searchsorted(ks, cell, side="right")
+                    for j in order[lo:hi]:
+                        if j == i:
+                            continue
+                        d = sc.pos[j] - sc.pos[i]
+                        if d @ d < (2 * r) ** 2 * (1 + 1e-10):
+                            out.append((i, j))
+    return np.array(out, np.uint32).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("bits", [6, 10, 14, 18, 20, 22, 25])
+def test_sort_and_candidates_at_every_digit_width(bits):
+    """Key widths 9 .. 25 bits: 1 pass of 8 or 11 bits, 2 of 8, 9 or 11,
+    3 of 9."""
+    from grains3dgpu_amd import engine
+    from oracle import pyoracle
+    sc = sparse_spheres(bits)
+    g = sc.sims[0]
+    n3 = pyoracle.cell_counts(g["domain_min"], g["domain_max"], 1e-3).astype(np.int64)
+    cells = int(np.prod(n3))
+    width = int(np.ceil(np.log2(cells)))
+    e = S.load(engine.Engine(dt=sc.dt, precision=A.FP64), sc)
+    e.step(1)
+    st = e.get_stats()
+    keys, sidx = e.get_cell_keys()
+    np.testing.assert_array_equal(sidx, np.argsort(keys, kind="stable"))
+    assert st["sort_passes"] == -(-width // 11), (width, st["sort_passes"])
+    pairs = e.get_pairs(A.PAIRS_CANDIDATE)
+    e.close()
+    assert len(pairs) > 50
+    np.testing.assert_array_equal(pairs, brute_candidates(sc, keys.astype(np.int64), tuple(n3)))
