@@ -1,0 +1,464 @@
+"""The reference's XML case files, read without xerces, and its driver loop
+with the RawData writer.
+
+    case = read_case("deck.xml")          # a Case: Scene + time setting + writers
+    run_case(case, engine)                # steps it, writing RawData files
+
+`read_case` follows Grains<T>::Construction, Forces and AdditionalFeatures
+(Grains/Base/src/Grains.cpp:91-401) element by element:
+
+- LinkedCell MX/MY/MZ (the domain's maximum corner), CellSizeFactor (cell =
+  factor x 2 x the largest circumscribed radius of the particles, factor
+  clamped to >= 1), Origin OX/OY/OZ (default 0), Periodicity (read and
+  ignored: the reference sets m_isPeriodic = false and exits only when all
+  three are set, Grains.cpp:120-133);
+- Particles: every Particle element, Number copies of one rigid body
+  (RigidBody.cpp:64-100): Convex with CrustThickness and one of Sphere
+  (Radius), Box (LX, LY, LZ: full lengths, halved), Superquadric (a, b, c,
+  n1, n2), Rectangle (LX, LY: halved); Density; Material (names numbered in
+  order of first appearance, particles first, then obstacles); the node's
+  Centre and AngularPosition (Transform3.cpp: Type "Matrix" = 9 row-major
+  values, "Angles" = aX, aY, aZ in degrees through setBasis, anything else
+  the identity);
+- Obstacles: every Obstacle element, Composite or not, the same way;
+- ContactForceModels: ContactForceModel Type="Hooke", Material materialA /
+  materialB, Parameters kn, en, etat, muc, kr
+  (ContactForceModelBuilderFactory.cpp:88-121, HookeContactForceModel.cpp:
+  15-45), indexed by computeHash;
+- TemporalSetting: TimeInterval Start, End, dt; TimeIntegration
+  FirstOrderExplicit;
+- Forces: Gravity GX, GY, GZ;
+- Simulation: ParticleInsertion (InitialPosition, InitialOrientation,
+  InitialVelocity, InitialAngularVelocity, each Type Zero | Constant X Y Z |
+  File Name | Random with Box windows; Insertion.cpp), PostProcessing
+  TimeSave Start/End/dt and Writers RawData Directory/RootName.
+
+Differences from the reference, each where the reference is undefined or
+cannot run:
+- a Material given as a child element (<Material>matW</Material>, used by
+  obstacle decks) is accepted; the reference reads only the attribute and
+  dereferences a null node without it (ReaderXML.cpp:91-98);
+- a material's first rigid body gets its material's id; the reference
+  leaves m_material unset for the body that introduces a new name
+  (RigidBody.cpp:78-88);
+- Random insertion draws uniformly in the first window with numpy's PCG64
+  (seed 0 for "Default", the Value attribute for "UserDefined", fresh
+  entropy for "Random"); the reference's box window shadows its corners with
+  locals (InsertionWindow.cpp:33-43), so every point it draws is the origin;
+- contact models of other types (the legacy HODC blocks of older decks) are
+  rejected with CaseError, as the reference exits on them.
+
+`run_case` reproduces GrainsCPU<T>::simulate (GrainsCPU.cpp:46-70): insert
+(basis = R_insert R_particle, origin = the inserted point,
+ComponentManagerCPU.cpp:257-273), then for t = tStart; t <= tEnd + HIGHEPS;
+t += dt (in the case's precision) one step, after which the writers fire
+when the next save time is within 0.01 dt of t (Grains.cpp:58-72); save
+times are Start, Start + dt, ... <= End, then End once more.
+"""
+from __future__ import annotations
+
+import math
+import os
+import xml.etree.ElementTree as ET
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+from .scene import Scene, SHAPE_DTYPE
+
+HIGHEPS = {A.FP64: 1e-15, A.FP32_MIXED: 1e-8}  # Basic.hh:116-118
+
+
+class CaseError(ValueError):
+    pass
+
+
+@dataclass
+class Writer:
+    kind: str
+    directory: str
+    root: str
+
+
+@dataclass
+class Case:
+    scene: Scene
+    precision: int
+    t_start: float
+    t_end: float
+    dt: float
+    save_times: list
+    writers: list
+    materials: dict
+    insertion: dict = field(default_factory=dict)
+    particle_tr: np.ndarray | None = None  # per particle: the Particle node's transform (12)
+
+
+def _f(node, name, default=None):
+    v = node.get(name)
+    if v is None:
+        if default is None:
+            raise CaseError(f"<{node.tag}> needs attribute {name}")
+        return default
+    return float(v)
+
+
+def _child(node, tag, required=False):
+    c = node.find(tag)
+    if c is None and required:
+        raise CaseError(f"<{node.tag}> needs <{tag}>")
+    return c
+
+
+def _rot_angles(ax, ay, az):
+    """Transform3::setBasis(aX, aY, aZ) (Transform3.cpp:151-162), row major."""
+    cx, sx, cy, sy, cz, sz = math.cos(ax), math.sin(ax), math.cos(ay), math.sin(ay), math.cos(az), math.sin(az)
+    return np.array([cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx,
+                     sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx,
+                     -sy, cy * sx, cy * cx])
+
+
+def _transform(node):
+    """Transform3(DOMNode*): Centre and AngularPosition (Transform3.cpp)."""
+    t = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], float)
+    c = node.find("Centre")
+    if c is not None:
+        t[9:] = [_f(c, "X"), _f(c, "Y"), _f(c, "Z")]
+    ap = node.find("AngularPosition")
+    kind = ap.get("Type") if ap is not None else None
+    if kind == "Matrix":
+        vals = [float(x) for x in (ap.text or "").split()]
+        if len(vals) != 9:
+            raise CaseError("AngularPosition Type=\"Matrix\" needs 9 values")
+        R = np.array(vals).reshape(3, 3)
+        if not (np.allclose(R @ R.T, np.eye(3), atol=1e-6) and np.linalg.det(R) > 0):
+            raise CaseError("an AngularPosition matrix is not a rotation matrix")
+        t[:9] = vals
+    elif kind == "Angles":
+        d = math.pi / 180.0
+        t[:9] = _rot_angles(d * _f(ap, "aX"), d * _f(ap, "aY"), d * _f(ap, "aZ"))
+    return t
+
+
+def _material(node):
+    m = node.get("Material")
+    if m is None:
+        c = node.find("Material")
+        if c is None or not (c.text or "").strip():
+            raise CaseError(f"<{node.tag} {node.attrib}> has no Material")
+        m = c.text.strip()
+    return m
+
+
+def _shape(node, materials):
+    """A rigid body's shape record (grains_shape_t) and vertices: RigidBody
+    ctor (RigidBody.cpp:64-100) with ConvexBuilderFactory::create."""
+    cv = _child(node, "Convex", True)
+    kids = [k for k in cv if isinstance(k.tag, str)]
+    if not kids:
+        raise CaseError("<Convex> needs a shape element")
+    el = kids[0]
+    rec = np.zeros(1, SHAPE_DTYPE)[0]
+    rec["crust"] = _f(cv, "CrustThickness")
+    rec["density"] = _f(node, "Density", 0.0)
+    name = _material(node)
+    if name not in materials:
+        materials[name] = len(materials)
+    rec["material"] = materials[name]
+    if el.tag == "Sphere":
+        rec["kind"] = A.SPHERE
+        rec["dims"][0] = _f(el, "Radius")
+    elif el.tag == "Box":
+        rec["kind"] = A.BOX
+        rec["dims"][:3] = [_f(el, "LX") / 2, _f(el, "LY") / 2, _f(el, "LZ") / 2]
+    elif el.tag == "Superquadric":
+        rec["kind"] = A.SUPERQUADRIC
+        rec["dims"][:5] = [_f(el, k) for k in ("a", "b", "c", "n1", "n2")]
+    elif el.tag == "Rectangle":
+        rec["kind"] = A.RECTANGLE
+        rec["dims"][:2] = [_f(el, "LX") / 2, _f(el, "LY") / 2]
+    else:
+        raise CaseError(f"convex {el.tag} is not supported on this path")
+    return rec
+
+
+def _insertion_rule(node):
+    """One insertion policy (Insertion.cpp:8-160) as a dict."""
+    if node is None:
+        return dict(type="Zero")
+    kind = node.get("Type")
+    if kind == "Zero":
+        return dict(type="Zero")
+    if kind == "Constant":
+        return dict(type="Constant", value=[_f(node, "X"), _f(node, "Y"), _f(node, "Z")])
+    if kind == "File":
+        return dict(type="File", name=node.get("Name"))
+    if kind == "Random":
+        seed = node.get("Seed", "Default")
+        val = 0
+        if seed == "UserDefined":
+            val = int(node.get("Value", "0"))
+            if not val:
+                raise CaseError("Seed=\"UserDefined\" needs a Value")
+        elif seed == "Random":
+            val = None
+        wins = []
+        w = node.find("Windows")
+        for win in (list(w) if w is not None else []):
+            if not isinstance(win.tag, str):
+                continue
+            if win.get("Type") != "Box":
+                raise CaseError(f"insertion window {win.get('Type')} is not supported")
+            lo, hi = win.find("MinPoint"), win.find("MaxPoint")
+            wins.append(([_f(lo, "X"), _f(lo, "Y"), _f(lo, "Z")], [_f(hi, "X"), _f(hi, "Y"), _f(hi, "Z")]))
+        if not wins:
+            raise CaseError("Random insertion needs a Box window")
+        return dict(type="Random", seed=val, windows=wins)
+    raise CaseError(f"unknown insertion Type {kind}")
+
+
+def read_case(path, base_dir=None):
+    """Parse a reference XML deck into a Case (see the module docstring)."""
+    tree = ET.parse(path)
+    root = tree.getroot()
+    base_dir = base_dir or os.path.dirname(os.path.abspath(path))
+    prec = A.FP32_MIXED if root.get("Precision", "Double") == "Single" else A.FP64
+    cons = _child(root, "Construction", True)
+    lc = _child(cons, "LinkedCell", True)
+    mx = [_f(lc, "MX"), _f(lc, "MY"), _f(lc, "MZ")]
+    org = cons.find("Origin")
+    lo = [_f(org, "OX"), _f(org, "OY"), _f(org, "OZ")] if org is not None else [0.0, 0.0, 0.0]
+    factor = max(1.0, float(lc.get("CellSizeFactor", "1")))
+    materials = {}
+    shapes, particle_shape, ptr = [], [], []
+    for p in root.iter("Particle"):
+        rec = _shape(p, materials)
+        shapes.append(rec)
+        tr = _transform(p)
+        for _ in range(int(p.get("Number", "0"))):
+            particle_shape.append(len(shapes) - 1)
+            ptr.append(tr)
+    obs_shape, obs_tr = [], []
+    for o in root.iter("Obstacle"):
+        rec = _shape(o, materials)
+        shapes.append(rec)
+        obs_shape.append(len(shapes) - 1)
+        obs_tr.append(_transform(o))
+    nm = max(1, len(materials))
+    tables = np.full((nm * (nm + 1) // 2, 5), np.nan)
+    cfm = cons.find("ContactForceModels")
+    for c in (cfm.iter("ContactForceModel") if cfm is not None else []):
+        if c.get("Type") != "Hooke":
+            raise CaseError(f"contact force model {c.get('Type')!r} is not known (Hooke only)")
+        m = _child(c, "Material", True)
+        a, b = m.get("materialA"), m.get("materialB")
+        if a not in materials or b not in materials:
+            raise CaseError(f"contact model for unknown materials {a}, {b}")
+        ia, ib = materials[a], materials[b]
+        s, l_ = min(ia, ib), max(ia, ib)
+        prm = _child(c, "Parameters", True)
+        row = []
+        for k in ("kn", "en", "etat", "muc", "kr"):
+            e = prm.find(k)
+            if e is None:
+                raise CaseError(f"Hooke parameters need <{k}>")
+            row.append(float(e.text))
+        tables[l_ * (l_ + 1) // 2 + s] = row
+    if np.isnan(tables).any():
+        names = {v: k for k, v in materials.items()}
+        miss = [i for i in range(len(tables)) if np.isnan(tables[i]).any()]
+        pairs = []
+        for i in miss:
+            l_ = int((math.isqrt(8 * i + 1) - 1) // 2)
+            pairs.append((names.get(i - l_ * (l_ + 1) // 2), names.get(l_)))
+        raise CaseError(f"no contact force model for material pairs {pairs}")
+    ts = _child(cons, "TemporalSetting", True)
+    ti = _child(ts, "TimeInterval", True)
+    integ = ts.find("TimeIntegration")
+    if integ is not None and integ.get("Type", "FirstOrderExplicit") != "FirstOrderExplicit":
+        raise CaseError(f"time integration {integ.get('Type')} is not supported (FirstOrderExplicit)")
+    g = [0.0, 0.0, 0.0]
+    forces = root.find("Forces")
+    if forces is not None:
+        gn = forces.find("Gravity")
+        if gn is None:
+            raise CaseError("Gravity is mandatory in <Forces>")
+        g = [_f(gn, "GX"), _f(gn, "GY"), _f(gn, "GZ")]
+    sim = _child(root, "Simulation", True)
+    ins = sim.find("ParticleInsertion")
+    insertion = {k: _insertion_rule(ins.find(k) if ins is not None else None)
+                 for k in ("InitialPosition", "InitialOrientation", "InitialVelocity", "InitialAngularVelocity")}
+    for rule in insertion.values():
+        if rule["type"] == "File":
+            rule["name"] = os.path.join(base_dir, rule["name"]) if not os.path.isabs(rule["name"]) else rule["name"]
+    save, writers = [], []
+    pp = sim.find("PostProcessing")
+    if pp is not None:
+        tsv = _child(pp, "TimeSave", True)
+        t0, t1, dts = _f(tsv, "Start"), _f(tsv, "End"), _f(tsv, "dt")
+        T = np.float32 if prec == A.FP32_MIXED else np.float64
+        t = T(t0)
+        while t <= T(t1):
+            save.append(float(t))
+            t = T(t + T(dts))
+        save.append(float(T(t1)))
+        ws = pp.find("Writers")
+        for w in (list(ws) if ws is not None else []):
+            if not isinstance(w.tag, str):
+                continue
+            if w.tag != "RawData":
+                raise CaseError(f"writer {w.tag} is not supported (RawData)")
+            d = w.get("Directory", ".")
+            writers.append(Writer("RawData", d if os.path.isabs(d) else os.path.join(base_dir, d),
+                                  w.get("RootName", "Results")))
+    n = len(particle_shape)
+    sc = Scene(shapes=np.array(shapes, SHAPE_DTYPE), tables=tables, n_materials=nm,
+               sims=[dict(domain_min=tuple(lo), domain_max=tuple(mx), gravity=tuple(g), cell_size=0.0,
+                          cell_size_factor=factor, periodic=0)],
+               particle_shape=np.array(particle_shape, np.uint32), pos=np.zeros((n, 3)),
+               quat=np.tile([0, 0, 0, 1.0], (n, 1)), vel=np.zeros((n, 3)), omega=np.zeros((n, 3)),
+               obstacle_shape=np.array(obs_shape, np.uint32),
+               obstacle_tr=np.array(obs_tr).reshape(-1, 12), dt=_f(ti, "dt"),
+               name=os.path.basename(path))
+    return Case(scene=sc, precision=prec, t_start=_f(ti, "Start"), t_end=_f(ti, "End"), dt=_f(ti, "dt"),
+                save_times=save, writers=writers, materials=materials, insertion=insertion,
+                particle_tr=np.array(ptr).reshape(-1, 12))
+
+
+def _draw(rule, n):
+    """n vectors from one insertion policy (fetchInsertionDataForEach,
+    Insertion.cpp:190-215)."""
+    if rule["type"] == "Zero":
+        return np.zeros((n, 3))
+    if rule["type"] == "Constant":
+        return np.tile(rule["value"], (n, 1))
+    if rule["type"] == "File":
+        vals = np.array(open(rule["name"]).read().split(), float)
+        if vals.size < 3 * n:
+            raise CaseError(f"{rule['name']} holds {vals.size // 3} vectors, {n} needed")
+        return vals[:3 * n].reshape(n, 3)
+    lo, hi = (np.array(x, float) for x in rule["windows"][0])
+    rng = np.random.default_rng(rule["seed"])
+    return lo + rng.random((n, 3)) * (hi - lo)
+
+
+def insert(case):
+    """Initial state after insertParticles (ComponentManagerCPU.cpp:257-273):
+    basis = setBasis(orientation angles) x the Particle node's basis, origin =
+    the inserted position; velocities from the two velocity policies."""
+    sc = case.scene
+    n = sc.n
+    pos = _draw(case.insertion["InitialPosition"], n)
+    ang = _draw(case.insertion["InitialOrientation"], n)
+    vel = _draw(case.insertion["InitialVelocity"], n)
+    om = _draw(case.insertion["InitialAngularVelocity"], n)
+    quat = np.zeros((n, 4))
+    for i in range(n):
+        R = _rot_angles(*ang[i]).reshape(3, 3) @ case.particle_tr[i, :9].reshape(3, 3)
+        quat[i] = _quat_of(R)
+    sc.pos, sc.quat, sc.vel, sc.omega = pos, quat, vel, om
+    return sc
+
+
+def _quat_of(R):
+    """Unit quaternion (x, y, z, w) of a rotation matrix (the largest of the
+    four diagonal combinations, as Quaternion(Matrix3), Quaternion.cpp:
+    125-169)."""
+    t = np.trace(R)
+    if t > 0:
+        s = math.sqrt(t + 1.0) * 2
+        q = [(R[2, 1] - R[1, 2]) / s, (R[0, 2] - R[2, 0]) / s, (R[1, 0] - R[0, 1]) / s, 0.25 * s]
+    elif R[0, 0] > R[1, 1] and R[0, 0] > R[2, 2]:
+        s = math.sqrt(1.0 + R[0, 0] - R[1, 1] - R[2, 2]) * 2
+        q = [0.25 * s, (R[0, 1] + R[1, 0]) / s, (R[0, 2] + R[2, 0]) / s, (R[2, 1] - R[1, 2]) / s]
+    elif R[1, 1] > R[2, 2]:
+        s = math.sqrt(1.0 + R[1, 1] - R[0, 0] - R[2, 2]) * 2
+        q = [(R[0, 1] + R[1, 0]) / s, 0.25 * s, (R[1, 2] + R[2, 1]) / s, (R[0, 2] - R[2, 0]) / s]
+    else:
+        s = math.sqrt(1.0 + R[2, 2] - R[0, 0] - R[1, 1]) * 2
+        q = [(R[0, 2] + R[2, 0]) / s, (R[1, 2] + R[2, 1]) / s, 0.25 * s, (R[1, 0] - R[0, 1]) / s]
+    q = np.array(q)
+    return q / np.linalg.norm(q)
+
+
+class RawDataWriter:
+    """RawDataPostProcessingWriter (PostProcessing/src/RawDataPostProcessingWriter.cpp):
+    one line per save -- the time, then each particle's value, then each
+    obstacle's -- in <root>_position_{x,y,z}.dat,
+    <root>_translational_velocity_{x,y,z}.dat,
+    <root>_angular_velocity_{x,y,z}.dat, all in scientific notation with 6
+    digits; <root>_coordinationNumber.dat gets an empty line per save and
+    <root>_particleType.dat is rewritten with an empty line (the reference
+    writes neither's values)."""
+    FILES = [f"{q}_{a}" for q in ("position", "translational_velocity", "angular_velocity") for a in "xyz"]
+
+    def __init__(self, directory, root):
+        self.directory, self.root = directory, root
+        self.fh = {}
+
+    def path(self, name):
+        return os.path.join(self.directory, f"{self.root}_{name}.dat")
+
+    def start(self):
+        os.makedirs(self.directory, exist_ok=True)
+        for f in self.FILES + ["coordinationNumber"]:
+            self.fh[f] = open(self.path(f), "w")
+
+    @staticmethod
+    def fmt(x):
+        return f"{x:.6e}"
+
+    def write(self, t, pos, vel, omega, obstacle_pos):
+        no = len(obstacle_pos)
+        zo = np.zeros((no, 3))
+        cols = {"position": np.concatenate([pos, obstacle_pos]), "translational_velocity": np.concatenate([vel, zo]),
+                "angular_velocity": np.concatenate([omega, zo])}
+        st = self.fmt(t)
+        for q, arr in cols.items():
+            for k, a in enumerate("xyz"):
+                self.fh[f"{q}_{a}"].write(st + "".join(" " + self.fmt(v) for v in arr[:, k]) + "\n")
+        self.fh["coordinationNumber"].write("\n")
+        with open(self.path("particleType"), "w") as f:
+            f.write("\n")
+
+    def end(self):
+        for f in self.fh.values():
+            f.close()
+        self.fh = {}
+
+
+def run_case(case, backend, log=None):
+    """Insert, then march the reference's time loop on `backend` (an Engine,
+    or any object with the grains_set_* / step / get_state calls), firing
+    the writers at the save times.  Returns the number of steps taken."""
+    from .scene import load
+    sc = insert(case)
+    load(backend, sc)
+    writers = [RawDataWriter(w.directory, w.root) for w in case.writers]
+    for w in writers:
+        w.start()
+    T = np.float32 if case.precision == A.FP32_MIXED else np.float64
+    t, t_end, dt = T(case.t_start), T(T(case.t_end) + T(HIGHEPS[case.precision])), T(case.dt)
+    saves = list(case.save_times)
+    obs_pos = np.asarray(case.scene.obstacle_tr).reshape(-1, 12)[:, 9:]
+    steps, pending = 0, 0
+    while t <= t_end:
+        pending += 1
+        if saves and T(saves[0]) - t < T(0.01) * dt:
+            backend.step(pending)
+            steps += pending
+            pending = 0
+            saves.pop(0)
+            p, _, v, w = backend.get_state()
+            for wr in writers:
+                wr.write(float(t), p, v, w, obs_pos)
+            if log:
+                log(f"t = {float(t):.6e}: {steps} steps")
+        t = T(t + dt)
+    if pending:
+        backend.step(pending)
+        steps += pending
+    for w in writers:
+        w.end()
+    return steps

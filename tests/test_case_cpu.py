@@ -1,0 +1,201 @@
+"""The XML case reader, the insertion policies and the RawData writer
+(grains3dgpu_amd/case.py), on the CPU: decks in the current format of the
+reference (Grains.cpp:91-401), the reference's own test decks (legacy
+contact blocks, which the reference rejects), and a run through the FP64
+oracle with the reference's driver loop."""
+import os
+
+import numpy as np
+import pytest
+
+from grains3dgpu_amd import _abi as A
+from grains3dgpu_amd import case as C
+from grains3dgpu_amd import scene as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DECKS = os.path.join(HERE, "golden", "reference_decks")
+
+DECK = """<Grains3D Type="HIP" Precision="Double">
+<Construction>
+  <LinkedCell MX="0.012" MY="0.012" MZ="0.024" CellSizeFactor="1.5"/>
+  <Origin OX="0" OY="0" OZ="0"/>
+  <Periodicity PX="0" PY="0" PZ="0"/>
+  <Particles>
+    <Particle Number="3" Density="2500" Material="glass">
+      <Convex CrustThickness="5.e-5"><Sphere Radius="0.0005"/></Convex>
+      <AngularPosition Type="Identity"/>
+    </Particle>
+    <Particle Number="2" Density="2000" Material="sand">
+      <Convex CrustThickness="5.e-5"><Box LX="0.001" LY="0.0008" LZ="0.0006"/></Convex>
+      <AngularPosition Type="Angles" aX="30" aY="0" aZ="90"/>
+    </Particle>
+    <Particle Number="1" Density="2000" Material="sand">
+      <Convex CrustThickness="5.e-5"><Superquadric a="6e-4" b="5e-4" c="4e-4" n1="4" n2="3"/></Convex>
+    </Particle>
+  </Particles>
+  <Obstacles>
+    <Composite name="Box">
+      <Obstacle name="Floor" Material="steel">
+        <Convex CrustThickness="0.0"><Rectangle LX="0.012" LY="0.012"/></Convex>
+        <Centre X="0.006" Y="0.006" Z="0"/>
+        <AngularPosition Type="Matrix">1 0 0  0 1 0  0 0 1</AngularPosition>
+      </Obstacle>
+    </Composite>
+  </Obstacles>
+  <ContactForceModels>
+    <ContactForceModel Type="Hooke">
+      <Material materialA="glass" materialB="glass"/>
+      <Parameters><kn>1e5</kn><en>0.8</en><etat>2e4</etat><muc>0.3</muc><kr>0</kr></Parameters>
+    </ContactForceModel>
+    <ContactForceModel Type="Hooke">
+      <Material materialA="sand" materialB="glass"/>
+      <Parameters><kn>2e5</kn><en>0.7</en><etat>2e4</etat><muc>0.4</muc><kr>0</kr></Parameters>
+    </ContactForceModel>
+    <ContactForceModel Type="Hooke">
+      <Material materialA="sand" materialB="sand"/>
+      <Parameters><kn>3e5</kn><en>0.6</en><etat>2e4</etat><muc>0.5</muc><kr>0</kr></Parameters>
+    </ContactForceModel>
+    <ContactForceModel Type="Hooke">
+      <Material materialA="glass" materialB="steel"/>
+      <Parameters><kn>4e5</kn><en>0.9</en><etat>2e4</etat><muc>0.2</muc><kr>0</kr></Parameters>
+    </ContactForceModel>
+    <ContactForceModel Type="Hooke">
+      <Material materialA="steel" materialB="sand"/>
+      <Parameters><kn>5e5</kn><en>0.9</en><etat>2e4</etat><muc>0.2</muc><kr>0</kr></Parameters>
+    </ContactForceModel>
+    <ContactForceModel Type="Hooke">
+      <Material materialA="steel" materialB="steel"/>
+      <Parameters><kn>6e5</kn><en>0.9</en><etat>2e4</etat><muc>0.2</muc><kr>0</kr></Parameters>
+    </ContactForceModel>
+  </ContactForceModels>
+  <TemporalSetting>
+    <TimeInterval Start="0." End="2.e-5" dt="1.e-6"/>
+    <TimeIntegration Type="FirstOrderExplicit"/>
+  </TemporalSetting>
+</Construction>
+<Forces><Gravity GX="0.0" GY="0.0" GZ="-9.81"/></Forces>
+<Simulation>
+  <ParticleInsertion>
+    <InitialPosition Type="File" Name="pos.dat"/>
+    <InitialOrientation Type="Zero"/>
+    <InitialVelocity Type="Constant" X="0" Y="0" Z="-0.5"/>
+    <InitialAngularVelocity Type="Zero"/>
+  </ParticleInsertion>
+  <PostProcessing>
+    <TimeSave Start="0." End="2.e-5" dt="5.e-6"/>
+    <Writers><RawData Directory="out" RootName="Results"/></Writers>
+  </PostProcessing>
+</Simulation>
+</Grains3D>
+"""
+
+POS = [[0.003, 0.003, 0.00049], [0.0041, 0.003, 0.002], [0.006, 0.006, 0.004],
+       [0.008, 0.003, 0.002], [0.0089, 0.0031, 0.0031], [0.003, 0.008, 0.003]]
+
+
+@pytest.fixture
+def deck(tmp_path):
+    (tmp_path / "deck.xml").write_text(DECK)
+    (tmp_path / "pos.dat").write_text("\n".join(" ".join(repr(v) for v in p) for p in POS) + "\n")
+    return tmp_path / "deck.xml"
+
+
+def test_read_case_fields(deck):
+    c = C.read_case(str(deck))
+    sc = c.scene
+    assert c.precision == A.FP64 and sc.n == 6 and c.dt == 1e-6
+    # materials in order of first appearance, particles then obstacles
+    assert c.materials == {"glass": 0, "sand": 1, "steel": 2}
+    sh = sc.shapes
+    assert sh["kind"].tolist() == [A.SPHERE, A.BOX, A.SUPERQUADRIC, A.RECTANGLE]
+    np.testing.assert_array_equal(sh["dims"][1][:3], [5e-4, 4e-4, 3e-4])  # Box LX/2 ...
+    np.testing.assert_array_equal(sh["dims"][3][:2], [6e-3, 6e-3])        # Rectangle LX/2, LY/2
+    assert sc.particle_shape.tolist() == [0, 0, 0, 1, 1, 2]
+    # computeHash(l(l+1)/2 + s): (0,0) (0,1) (1,1) (0,2) (1,2) (2,2)
+    np.testing.assert_array_equal(sc.tables[:, 0], [1e5, 2e5, 3e5, 4e5, 5e5, 6e5])
+    g = sc.sims[0]
+    assert g["domain_max"] == (0.012, 0.012, 0.024) and g["cell_size_factor"] == 1.5
+    assert g["gravity"] == (0.0, 0.0, -9.81)
+    # Angles in degrees through setBasis (Transform3.cpp:151-162)
+    R = c.particle_tr[3, :9].reshape(3, 3)
+    np.testing.assert_allclose(R, C._rot_angles(np.pi / 6, 0, np.pi / 2).reshape(3, 3), atol=0)
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-15)
+    # save times: Start, Start + dt, ... <= End, then End again
+    np.testing.assert_allclose(c.save_times, [0, 5e-6, 1e-5, 1.5e-5, 2e-5, 2e-5], rtol=1e-12)
+    np.testing.assert_array_equal(sc.obstacle_tr[0, 9:], [0.006, 0.006, 0.0])
+
+
+def test_insertion_policies(deck):
+    c = C.read_case(str(deck))
+    sc = C.insert(c)
+    np.testing.assert_array_equal(sc.pos, POS)
+    np.testing.assert_array_equal(sc.vel, np.tile([0, 0, -0.5], (6, 1)))
+    # orientation: setBasis(0, 0, 0) x the Particle node's basis
+    for i in range(6):
+        np.testing.assert_allclose(S.quat_to_matrix(sc.quat[i]), c.particle_tr[i, :9], atol=1e-15)
+
+
+def test_quaternion_of_matrix_round_trip():
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        q = rng.standard_normal(4)
+        q /= np.linalg.norm(q)
+        R = S.quat_to_matrix(q).reshape(3, 3)
+        np.testing.assert_allclose(S.quat_to_matrix(C._quat_of(R)).reshape(3, 3), R, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["Cinsert", "Ginsert", "Dinsert"])
+def test_reference_test_decks_use_legacy_contact_blocks(name):
+    """The reference's own decks carry HODC contact blocks without a Type;
+    the reference's factory exits on them (ContactForceModelBuilderFactory.cpp:
+    106-112), and so does the reader."""
+    with pytest.raises(C.CaseError, match="not known"):
+        C.read_case(os.path.join(DECKS, name + ".xml"))
+
+
+def test_random_window_insertion(tmp_path):
+    deck = DECK.replace('<InitialPosition Type="File" Name="pos.dat"/>',
+                        '<InitialPosition Type="Random" Seed="UserDefined" Value="7"><Windows>'
+                        '<Window Type="Box"><MinPoint X="0.001" Y="0.002" Z="0.003"/>'
+                        '<MaxPoint X="0.004" Y="0.005" Z="0.006"/></Window></Windows></InitialPosition>')
+    (tmp_path / "d.xml").write_text(deck)
+    sc = C.insert(C.read_case(str(tmp_path / "d.xml")))
+    assert (sc.pos >= [0.001, 0.002, 0.003]).all() and (sc.pos <= [0.004, 0.005, 0.006]).all()
+    sc2 = C.insert(C.read_case(str(tmp_path / "d.xml")))
+    np.testing.assert_array_equal(sc.pos, sc2.pos)  # UserDefined seeds repeat
+
+
+def test_missing_contact_model_is_an_error(tmp_path):
+    deck = DECK.replace('<Material materialA="steel" materialB="steel"/>', '<Material materialA="glass" materialB="glass"/>')
+    (tmp_path / "d.xml").write_text(deck)
+    with pytest.raises(C.CaseError, match="steel"):
+        C.read_case(str(tmp_path / "d.xml"))
+
+
+def test_run_case_through_the_oracle_writes_rawdata(deck):
+    """The reference loop (t from 0 while t <= End + HIGHEPS: 21 steps for
+    End = 2e-5, dt = 1e-6), writers after the steps that start at t = 0,
+    5e-6, ..., 2e-5 (End's second entry would fire on a 22nd step, which the
+    loop never takes); every line = the time, then the particles' values,
+    then the obstacle's, '%.6e'."""
+    from oracle import pyoracle
+    c = C.read_case(str(deck))
+    o = pyoracle.Oracle(dt=c.dt)
+    steps = C.run_case(c, o)
+    assert steps == 21
+    out = os.path.join(os.path.dirname(str(deck)), "out")
+    lines = open(os.path.join(out, "Results_position_z.dat")).read().splitlines()
+    assert len(lines) == 5
+    cols = [ln.split() for ln in lines]
+    assert all(len(cl) == 1 + 6 + 1 for cl in cols)
+    assert [cl[0] for cl in cols] == ["0.000000e+00", "5.000000e-06", "1.000000e-05", "1.500000e-05",
+                                      "2.000000e-05"]
+    p, _, v, _ = o.get_state()
+    np.testing.assert_array_equal([float(x) for x in cols[-1][1:7]], [float(f"{z:.6e}") for z in p[:, 2]])
+    assert cols[-1][7] == "0.000000e+00"  # the floor's centre z
+    vz = open(os.path.join(out, "Results_translational_velocity_z.dat")).read().splitlines()[-1].split()
+    np.testing.assert_array_equal([float(x) for x in vz[1:7]], [float(f"{z:.6e}") for z in v[:, 2]])
+    # the first sphere starts 10 um into the floor: it is pushed up
+    assert p[0, 2] > POS[0][2]
+    assert open(os.path.join(out, "Results_particleType.dat")).read() == "\n"
+    assert open(os.path.join(out, "Results_coordinationNumber.dat")).read() == "\n" * 5
