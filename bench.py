@@ -260,7 +260,8 @@ def pos_error(p32, p64, pos0):
 
 def step_stats(st):
     return {k: st[k] for k in ("candidate_pairs", "gjk_calls", "gjk_iterations", "contacts", "wall_contacts",
-                               "fp64_refines", "max_candidates", "sort_passes", "out_of_domain")}
+                               "fp64_refines", "max_candidates", "sort_passes", "out_of_domain",
+                               "narrow_deferred")}
 
 
 def oracle_rate(sc, seconds, threads=1):

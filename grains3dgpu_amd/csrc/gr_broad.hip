@@ -607,6 +607,23 @@ __global__ void k_expand(uint64_t n, const uint32_t* __restrict__ cand, const ui
   count_add(counters, kCntCandidates, c);
 }
 
+// ---- the screening pass's queue ------------------------------------------------
+// Block b copies its segment of the screen's queue (defer_count[b] entries
+// at defer_list + b * defer_cap) to defer_flat[defer_off[b] ...], coalesced:
+// the complete pass then reads entry k directly instead of searching for
+// its segment.
+__global__ void __launch_bounds__(kBlock) k_defer_flatten(DevScene d) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t cnt = d.defer_count[b], off = d.defer_off[b];
+  const uint32_t* seg = d.defer_list + (uint64_t)b * d.defer_cap;
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) d.defer_flat[off + k] = seg[k];
+}
+
+void launch_defer_flatten(const DevScene& d, hipStream_t s) {
+  launch_scan(d.defer_count, d.defer_off, d.narrow_blocks, d.scan_blocks, s);
+  hipLaunchKernelGGL(k_defer_flatten, dim3(d.narrow_blocks), dim3(kBlock), 0, s, d);
+}
+
 void launch_expand(const DevScene& d, hipStream_t s) {
   hipLaunchKernelGGL(k_expand, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d.n, d.cand,
                      d.cand_count, d.entry_off, d.pair_i, d.pair_j, d.counters);

@@ -713,6 +713,7 @@ static int finalize(grains_ctx* ctx) {
       (rc = dalloc(ctx, &d.pair_res, ecap * res_sz, L)) || (rc = dalloc(ctx, &d.refine_list, ecap * 4, L)) ||
       (rc = dalloc(ctx, &d.refine_count, 16, L)) || (rc = dalloc(ctx, &d.defer_list, (ecap + max_nb * 256) * 4, L)) ||
       (rc = dalloc(ctx, &d.defer_count, max_nb * 4, L)) || (rc = dalloc(ctx, &d.defer_off, (max_nb + 1) * 4, L)) ||
+      (rc = dalloc(ctx, &d.defer_flat, (ecap + 1) * 4, L)) ||
       (rc = dalloc(ctx, &d.counters, (size_t)kNumCounters * kCntSlots * 8, L)))
     return rc;
   GR_HIP(hipMemcpy(p64, np64.data(), S * sizeof(ShapeNP<double>), hipMemcpyHostToDevice));
@@ -934,6 +935,7 @@ static int read_stats(grains_ctx* ctx, uint32_t steps) {
   st.out_of_domain = c[kCntOod];
   st.deep_contacts = c[kCntDeep];
   st.fp64_refines = c[kCntRefine];
+  st.narrow_deferred = c[kCntDeferred];
   st.neighbor_overflow = c[kCntOverflow];
   st.max_candidates = (uint32_t)c[kCntMaxCand];
   st.sort_passes = (uint32_t)ctx->passes;

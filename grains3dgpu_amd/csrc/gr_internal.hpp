@@ -57,6 +57,7 @@ enum Counter {
   kCntOverflow,
   kCntMaxCand,   // stored as max
   kCntBackup,
+  kCntDeferred,  // entries the screening pass queued
   kNumCounters = 16
 };
 
@@ -135,6 +136,7 @@ struct DevScene {
                                 // segment of defer_cap words per block
   uint32_t* defer_count;        // per block of the screen (narrow_blocks)
   uint32_t* defer_off;          // exclusive scan of defer_count (+ total)
+  uint32_t* defer_flat;         // the segments flattened in order (the complete pass's input)
   uint32_t defer_cap;
   int32_t split;                // 1: screening pass + pass over the queued entries
   int32_t cand_staged;          // 1: candidate search stages neighbour windows in LDS
@@ -179,6 +181,9 @@ void launch_sort(DevScene& d, int passes, int digit_bits, uint32_t** keys_sorted
 void launch_cells(const DevScene& d, const uint32_t* keys_s, const uint32_t* idx_s, hipStream_t s);
 void launch_candidates(const DevScene& d, const uint32_t* keys_s, hipStream_t s);
 void launch_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* block_sums, hipStream_t s);
+// the screening pass's queue: scan of the segment counts, then the segments
+// copied into defer_flat in order
+void launch_defer_flatten(const DevScene& d, hipStream_t s);
 void launch_expand(const DevScene& d, hipStream_t s);
 int cu_count();
 // gr_narrow64.hip / gr_narrow32.hip

@@ -111,8 +111,8 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
 // those of PASS 0 bit for bit.
 // The screen queues into one segment per block (an LDS counter, no global
 // atomic: a single queue counter would serialise one atomic per wave); the
-// segments are flattened by an exclusive scan of their counts, and PASS 1
-// finds an entry's segment by binary search over the scan.
+// segments are copied into one list at the offsets of an exclusive scan of
+// their counts (launch_defer_flatten), which PASS 1 reads in order.
 template <class T, bool SV, int PASS>
 __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   constexpr bool screen = SV && PASS == 2;
@@ -128,17 +128,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride) {
     uint32_t e = (uint32_t)k;
-    if constexpr (PASS == 1) {
-      uint32_t lo = 0, hi = gridDim.x;  // last segment b with defer_off[b] <= k
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (d.defer_off[mid] <= (uint32_t)k)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      e = d.defer_list[(uint64_t)lo * d.defer_cap + ((uint32_t)k - d.defer_off[lo])];
-    }
+    if constexpr (PASS == 1) e = d.defer_flat[k];
     NpStats st = {0, 0, 0, 0, 0};
     PairRes<T> r;
     uint32_t flag = pair_force<T, T, SV, screen>(d, d.pair_i[e], d.pair_j[e], r, st);
@@ -163,6 +153,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   if constexpr (screen) {
     __syncthreads();
     if (threadIdx.x == 0) d.defer_count[blockIdx.x] = seg_n;
+    count_add(d.counters, kCntDeferred, threadIdx.x == 0 ? seg_n : 0u);  // every lane: a wave reduction
   }
   count_add(d.counters, kCntGjk, c_gjk);
   count_add(d.counters, kCntGjkIters, c_it);

@@ -15,7 +15,7 @@ void launch_narrow32(const DevScene& d, int pass, hipStream_t s) {
     else
       hipLaunchKernelGGL((k_narrow<float, false, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
   } else if (two) {
-    launch_scan(d.defer_count, d.defer_off, d.narrow_blocks, d.scan_blocks, s);
+    launch_defer_flatten(d, s);
     hipLaunchKernelGGL((k_narrow<float, true, 1>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
   }
 }

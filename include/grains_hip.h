@@ -152,6 +152,8 @@ typedef struct grains_stats_t {
   uint32_t max_candidates;    /* largest per-particle candidate count seen    */
   uint32_t sort_passes;       /* radix passes per step                         */
   double   elapsed_ms;        /* device time of the last grains_step call      */
+  uint64_t narrow_deferred;   /* entries the screening pass queued for the
+                                 complete narrow pass                          */
 } grains_stats_t;
 
 typedef struct grains_ctx grains_ctx_t;
