@@ -81,6 +81,9 @@ def log(*a):
 GJK_TOL = {A.FP64: 1e-10, A.FP32_MIXED: 1e-6}
 
 
+MAX_STATS = ("max_candidates", "gjk_max_iterations", "sort_passes")
+
+
 def run_engine(sc, precision, warmup, steps, profile=True, flags=0, gjk_tol=None, graph=True, profile_steps=100,
                max_neighbors=64, **launch):
     """W warm-up steps, then (profile) min(profile_steps, steps) eagerly
@@ -111,7 +114,7 @@ def run_engine(sc, precision, warmup, steps, profile=True, flags=0, gjk_tol=None
         e.step(k, graph=graph)
         wall += time.perf_counter() - t0
         st = e.get_stats()
-        stats = st if stats is None else {c: (max(stats[c], st[c]) if c == "max_candidates" else stats[c] + st[c])
+        stats = st if stats is None else {c: (max(stats[c], st[c]) if c in MAX_STATS else stats[c] + st[c])
                                           for c in st}
         done += k
         log(f"    {done}/{steps} steps, {wall:.2f} s")
@@ -261,7 +264,7 @@ def pos_error(p32, p64, pos0):
 def step_stats(st):
     return {k: st[k] for k in ("candidate_pairs", "gjk_calls", "gjk_iterations", "contacts", "wall_contacts",
                                "fp64_refines", "max_candidates", "sort_passes", "out_of_domain",
-                               "narrow_deferred")}
+                               "narrow_deferred", "gjk_max_iterations")}
 
 
 def oracle_rate(sc, seconds, threads=1):

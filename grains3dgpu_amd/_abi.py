@@ -52,7 +52,8 @@ class Stats(C.Structure):
                 ("out_of_domain", C.c_uint64), ("deep_contacts", C.c_uint64),
                 ("fp64_refines", C.c_uint64), ("neighbor_overflow", C.c_uint64),
                 ("max_candidates", C.c_uint32), ("sort_passes", C.c_uint32),
-                ("elapsed_ms", C.c_double), ("narrow_deferred", C.c_uint64)]
+                ("elapsed_ms", C.c_double), ("narrow_deferred", C.c_uint64),
+                ("gjk_max_iterations", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

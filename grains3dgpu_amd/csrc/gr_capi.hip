@@ -922,7 +922,7 @@ static int read_stats(grains_ctx* ctx, uint32_t steps) {
     c[k] = 0;
     for (int j = 0; j < kCntSlots; ++j) {
       const unsigned long long v = slots[(size_t)k * kCntSlots + j];
-      c[k] = k == kCntMaxCand ? std::max(c[k], v) : c[k] + v;
+      c[k] = (k == kCntMaxCand || k == kCntMaxIters) ? std::max(c[k], v) : c[k] + v;
     }
   }
   grains_stats_t& st = ctx->stats;
@@ -936,6 +936,7 @@ static int read_stats(grains_ctx* ctx, uint32_t steps) {
   st.deep_contacts = c[kCntDeep];
   st.fp64_refines = c[kCntRefine];
   st.narrow_deferred = c[kCntDeferred];
+  st.gjk_max_iterations = c[kCntMaxIters];
   st.neighbor_overflow = c[kCntOverflow];
   st.max_candidates = (uint32_t)c[kCntMaxCand];
   st.sort_passes = (uint32_t)ctx->passes;

@@ -58,6 +58,7 @@ enum Counter {
   kCntMaxCand,   // stored as max
   kCntBackup,
   kCntDeferred,  // entries the screening pass queued
+  kCntMaxIters,  // stored as max: the longest GJK search
   kNumCounters = 16
 };
 

@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   }
   PairRes<T>* res = (PairRes<T>*)d.pair_res;
   const uint32_t E = PASS == 1 ? d.defer_off[gridDim.x] : d.entry_off[d.n];
-  uint32_t c_gjk = 0, c_it = 0, c_deep = 0, c_ref = 0, c_bk = 0;
+  uint32_t c_gjk = 0, c_it = 0, c_deep = 0, c_ref = 0, c_bk = 0, c_maxit = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < E; k += stride) {
     uint32_t e = (uint32_t)k;
@@ -143,6 +143,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
       d.pair_flag[e] = flag;
       c_gjk += st.gjk;
       c_it += st.iters;
+      c_maxit = c_maxit > (uint32_t)st.iters ? c_maxit : (uint32_t)st.iters;
       c_deep += st.deep;
       c_bk += st.backup;
       c_ref += refine;
@@ -160,6 +161,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   count_add(d.counters, kCntDeep, c_deep);
   count_add(d.counters, kCntRefine, c_ref);
   count_add(d.counters, kCntBackup, c_bk);
+  count_max(d.counters, kCntMaxIters, c_maxit);
 }
 
 // FP64 re-run of the entries queued by k_narrow<float>; results stored in FP32.

@@ -154,6 +154,7 @@ typedef struct grains_stats_t {
   double   elapsed_ms;        /* device time of the last grains_step call      */
   uint64_t narrow_deferred;   /* entries the screening pass queued for the
                                  complete narrow pass                          */
+  uint64_t gjk_max_iterations; /* the longest GJK search                       */
 } grains_stats_t;
 
 typedef struct grains_ctx grains_ctx_t;
