@@ -15,7 +15,8 @@ with the RawData writer.
 - Particles: every Particle element, Number copies of one rigid body
   (RigidBody.cpp:64-100): Convex with CrustThickness and one of Sphere
   (Radius), Box (LX, LY, LZ: full lengths, halved), Superquadric (a, b, c,
-  n1, n2), Rectangle (LX, LY: halved); Density; Material (names numbered in
+  n1, n2), Rectangle (LX, LY: halved), Cylinder and Cone (Radius, Height);
+  Density; Material (names numbered in
   order of first appearance, particles first, then obstacles); the node's
   Centre and AngularPosition (Transform3.cpp: Type "Matrix" = 9 row-major
   values, "Angles" = aX, aY, aZ in degrees through setBasis, anything else
@@ -178,6 +179,10 @@ def _shape(node, materials):
     elif el.tag == "Rectangle":
         rec["kind"] = A.RECTANGLE
         rec["dims"][:2] = [_f(el, "LX") / 2, _f(el, "LY") / 2]
+    elif el.tag in ("Cylinder", "Cone"):
+        # Radius and full Height (Cylinder.cpp:26-30, Cone.cpp:27-35)
+        rec["kind"] = A.CYLINDER if el.tag == "Cylinder" else A.CONE
+        rec["dims"][:2] = [_f(el, "Radius"), _f(el, "Height")]
     else:
         raise CaseError(f"convex {el.tag} is not supported on this path")
     return rec

@@ -25,6 +25,8 @@ struct HostShape {
   grains_shape_t s;
   double volume = 0, mass = 0, inertia[6] = {0}, inv[6] = {0};
   double rc = 0, rc_exact = 0, bbox[3] = {0}, core[5] = {0};
+  double bbox_exact[3] = {0};  // OBB half-extents containing the shape (physics semantics)
+  double gd[5] = {0};          // dims the support map reads (CYLINDER r, h/2; CONE r, h/4, sin)
   std::vector<double> verts, core_verts;
 };
 
@@ -159,6 +161,13 @@ void sym_inv(const double* I, double* r) {
   r[5] = (a * d - b * b) / det;
 }
 
+// Cone(r, h) members (Cone.cpp:9-14): r, h/4 and the sine of the half angle
+void cone_dims(double* g, double r, double q) {
+  g[0] = r;
+  g[1] = q;
+  g[2] = r / std::sqrt(r * r + 16.0 * q * q);
+}
+
 int derive_shape(HostShape& h, const grains_shape_t& s, const double* verts, uint32_t nverts) {
   h.s = s;
   const double* d = s.dims;
@@ -208,6 +217,35 @@ int derive_shape(HostShape& h, const grains_shape_t& s, const double* verts, uin
       h.bbox[0] = a, h.bbox[1] = b, h.bbox[2] = c;
       break;
     }
+    case GRAINS_CYLINDER: {  // Cylinder.cpp:9-12,72-111; dims: radius, height
+      double r = d[0], hh = d[1] / 2.0;
+      if (!(r > 0 && hh > 0)) return GRAINS_EINVAL;
+      h.volume = 2.0 * kPi * hh * r * r;
+      double r2 = r * r, c = 0.5 * kPi * hh * r2;
+      I[0] = I[5] = c * (4.0 * hh * hh / 3.0 + r2);
+      I[3] = 2.0 * c * r2;
+      Ii[5] = Ii[0] = 1.0 / I[0];
+      Ii[3] = 1.0 / I[3];
+      h.rc = std::sqrt(r * r + hh * hh);
+      h.bbox[0] = r, h.bbox[1] = hh, h.bbox[2] = r;
+      break;
+    }
+    case GRAINS_CONE: {  // Cone.cpp:9-14,76-122; dims: base radius, height
+      // origin at the centroid: apex at +3h/4 on y, base at -h/4
+      double r = d[0], q = d[1] / 4.0;
+      if (!(r > 0 && q > 0)) return GRAINS_EINVAL;
+      h.volume = 4.0 * q * kPi * r * r / 3.0;
+      double r2 = r * r, c = 0.2 * kPi * q * r2;
+      I[0] = I[5] = c * (4.0 * q * q + r2);
+      I[3] = 2.0 * c * r2;
+      Ii[5] = Ii[0] = 1.0 / I[0];
+      Ii[3] = 1.0 / I[3];
+      h.rc = std::max(std::sqrt(r * r + q * q), 3.0 * q);
+      // the reference box is centred on the centroid and leaves out the apex
+      // region y in (h/2, 3h/4] (Cone.cpp:116-122)
+      h.bbox[0] = r, h.bbox[1] = 2.0 * q, h.bbox[2] = r;
+      break;
+    }
     case GRAINS_RECTANGLE: {  // Rectangle.cpp:72-109
       double LX = d[0], LY = d[1];
       h.volume = 4.0 * LX * LY;
@@ -242,15 +280,27 @@ int derive_shape(HostShape& h, const grains_shape_t& s, const double* verts, uin
       return GRAINS_EUNSUPPORTED;
   }
   if (s.kind != GRAINS_SUPERQUADRIC) h.rc_exact = h.rc;
+  for (int k = 0; k < 3; ++k) h.bbox_exact[k] = h.bbox[k];
+  if (s.kind == GRAINS_CONE) h.bbox_exact[1] = 3.0 * d[1] / 4.0;
+  std::copy(d, d + 5, h.gd);
+  if (s.kind == GRAINS_CYLINDER) h.gd[1] = d[1] / 2.0;
+  if (s.kind == GRAINS_CONE) cone_dims(h.gd, d[0], d[1] / 4.0);
   h.mass = s.density * h.volume;
   for (int i = 0; i < 6; ++i) {
     h.inertia[i] = s.density == 0 ? 0.0 : I[i] * s.density;
     h.inv[i] = s.density == 0 ? 0.0 : Ii[i] / s.density;
   }
   // crust core: RigidBody::m_scaling (RigidBody.cpp:27-31)
-  std::copy(d, d + 5, h.core);
+  std::copy(h.gd, h.gd + 5, h.core);
   switch (s.kind) {
     case GRAINS_SPHERE: h.core[0] = d[0] - s.crust; break;
+    case GRAINS_CYLINDER:  // Cylinder(r - ct, h - 2 ct)
+      h.core[0] = h.gd[0] - s.crust;
+      h.core[1] = h.gd[1] - s.crust;
+      break;
+    case GRAINS_CONE:  // Cone(r - ct, h - 2 ct): y scaled by (h/2 - ct)/(h/2) about the centroid
+      cone_dims(h.core, h.gd[0] - s.crust, h.gd[1] - 0.5 * s.crust);
+      break;
     case GRAINS_BOX:
     case GRAINS_SUPERQUADRIC:
       for (int k = 0; k < 3; ++k) h.core[k] = d[k] - s.crust;
@@ -276,12 +326,12 @@ ShapeNP<T> to_np(const HostShape& h, bool quirks, int32_t voff) {
   r.nv = h.s.n_vertices;
   r.voff = voff;
   for (int k = 0; k < 5; ++k) {
-    r.dims[k] = T(h.s.dims[k]);
+    r.dims[k] = T(h.gd[k]);
     r.core[k] = T(h.core[k]);
   }
   r.crust = T(h.s.crust);
   r.rc = T(quirks ? h.rc : h.rc_exact);
-  for (int k = 0; k < 3; ++k) r.bbox[k] = T(h.bbox[k]);
+  for (int k = 0; k < 3; ++k) r.bbox[k] = T(quirks ? h.bbox[k] : h.bbox_exact[k]);
   r.mass = T(h.mass);
   return r;
 }

@@ -143,10 +143,18 @@ GR_D SupCtx<T> sup_ctx(const ShapeNP<T>& s, bool core, const T* verts_full, cons
 }
 
 // Sphere.cpp:111-114, Box.cpp:129-134, Rectangle.cpp:115-120,
-// Superquadric.cpp:169-205; polyhedron: first vertex maximising v.x.
+// Superquadric.cpp:169-205, Cylinder.cpp:118-131 (dims r, h/2),
+// Cone.cpp:126-141 (dims r, h/4, sine of the half angle); polyhedron: first
+// vertex maximising v.x.
+// The cylinder's and cone's test for a direction along the axis compares
+// |(v_x, v_z)| with EPS in the caller's length units; GJK directions are
+// differences of points, ~1e-6..1e-3 in SI for millimetre grains, so FP32
+// takes the FP64 value 1e-10 (FP32's 1e-5 would send most directions to the
+// axis point).
 template <class T>
 GR_D V3<T> support(const SupCtx<T>& c, V3<T> v) {
   const T* d = c.d;
+  constexpr T kAxisEps = T(Eps<double>::MID);
   switch (c.kind) {
     case GRAINS_SPHERE: {
       T n = norm(v);
@@ -161,6 +169,24 @@ GR_D V3<T> support(const SupCtx<T>& c, V3<T> v) {
         return sq_support_pow(d, v);
     case GRAINS_RECTANGLE:
       return {v.x < T(0) ? -d[0] : d[0], v.y < T(0) ? -d[1] : d[1], T(0)};
+    case GRAINS_CYLINDER: {
+      const T s = sqrt(v.x * v.x + v.z * v.z);
+      const T hy = v.y < T(0) ? -d[1] : d[1];
+      if (s > kAxisEps) {
+        const T k = d[0] / s;
+        return {v.x * k, hy, v.z * k};
+      }
+      return {T(0), hy, T(0)};
+    }
+    case GRAINS_CONE: {
+      if (v.y > norm(v) * d[2]) return {T(0), T(3) * d[1], T(0)};
+      const T s = sqrt(v.x * v.x + v.z * v.z);
+      if (s > kAxisEps) {
+        const T k = d[0] / s;
+        return {v.x * k, -d[1], v.z * k};
+      }
+      return {T(0), -d[1], T(0)};
+    }
     case GRAINS_POLYHEDRON: {
       const T* P = c.verts;
       V3<T> best = {P[0], P[1], P[2]};

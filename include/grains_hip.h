@@ -49,8 +49,9 @@ extern "C" {
 /* ---- convex kinds: numbering of ConvexType, Grains/Geometry/include/Convex.hh:7-16 */
 #define GRAINS_SPHERE 0
 #define GRAINS_BOX 1
-#define GRAINS_CYLINDER 2      /* not on this path: rejected with EUNSUPPORTED */
-#define GRAINS_CONE 3          /* not on this path: rejected with EUNSUPPORTED */
+#define GRAINS_CYLINDER 2      /* axis y, y in [-h/2, h/2] (Cylinder.cpp)    */
+#define GRAINS_CONE 3          /* axis y, origin at the centroid: apex +3h/4,
+                                  base -h/4 (Cone.cpp)                         */
 #define GRAINS_SUPERQUADRIC 4
 #define GRAINS_POLYHEDRON 5    /* enum value exists in the reference, class does not */
 #define GRAINS_RECTANGLE 6
@@ -83,7 +84,9 @@ typedef struct grains_shape_t {
   int32_t kind;          /* GRAINS_SPHERE ... GRAINS_RECTANGLE                */
   int32_t material;      /* material id; contact table index via computeHash  */
   double dims[5];        /* SPHERE r | BOX ex,ey,ez (half) | SUPERQUADRIC
-                            a,b,c,n1,n2 | RECTANGLE LX,LY (half)             */
+                            a,b,c,n1,n2 | RECTANGLE LX,LY (half) | CYLINDER
+                            r,h | CONE r,h (full height, as Cylinder(r, h),
+                            Cone(r, h))                                      */
   double crust;          /* CrustThickness                                    */
   double density;        /* 0 gives mass 0 (obstacles)                        */
   int32_t n_vertices;    /* POLYHEDRON: vertex count (convex hull points)     */

@@ -211,6 +211,8 @@ typedef struct {
   grains_shape_t s;
   double volume, mass, inertia[6], inv[6], rc, bbox[3], scaling[3];
   double rc_exact; /* true circumscribed radius (physics semantics) */
+  double bbox_exact[3]; /* OBB half-extents containing the shape (physics semantics) */
+  double gd[5];         /* dims the support map reads (CYLINDER r, h/2; CONE r, h/4, sin) */
   double core_dims[5];
   const double* verts; /* full shape vertices (polyhedron)          */
   double* core_verts;  /* vertices scaled to the crust core          */
@@ -381,8 +383,16 @@ static void sym_inverse(const double* I, double* inv) {
   inv[5] = (a * d - b * b) / det;
 }
 
+/* Cone(r, h) members (Cone.cpp:9-14): r, h/4 and the sine of the half angle */
+static void cone_dims(double* g, double r, double q) {
+  g[0] = r;
+  g[1] = q;
+  g[2] = r / sqrt(r * r + 16.0 * q * q);
+}
+
 /* RigidBody ctor (RigidBody.cpp:19-59) with the Convex virtuals of each shape:
- * Sphere.cpp:72-105, Box.cpp:81-123, Superquadric.cpp:81-163, Rectangle.cpp:72-109. */
+ * Sphere.cpp:72-105, Box.cpp:81-123, Superquadric.cpp:81-163, Rectangle.cpp:72-109,
+ * Cylinder.cpp:9-12,72-111, Cone.cpp:9-14,76-122. */
 static int shape_derive(oshape* o, const grains_shape_t* s, const double* verts) {
   memset(o, 0, sizeof(*o));
   o->s = *s;
@@ -430,6 +440,35 @@ static int shape_derive(oshape* o, const grains_shape_t* s, const double* verts)
     o->bbox[0] = a; o->bbox[1] = b; o->bbox[2] = c;
     break;
   }
+  case GRAINS_CYLINDER: {
+    /* dims: radius, height (Cylinder(r, h)); the support reads r, h/2 */
+    double r = d[0], hh = d[1] / 2.0;
+    o->volume = 2.0 * G_PI * hh * r * r;
+    double r2 = r * r, c = 0.5 * G_PI * hh * r2;
+    I[0] = I[5] = c * (4.0 * hh * hh / 3.0 + r2);
+    I[3] = 2.0 * c * r2;
+    Iinv[5] = Iinv[0] = 1.0 / I[0];
+    Iinv[3] = 1.0 / I[3];
+    o->rc = sqrt(r * r + hh * hh);
+    o->bbox[0] = r; o->bbox[1] = hh; o->bbox[2] = r;
+    break;
+  }
+  case GRAINS_CONE: {
+    /* dims: base radius, height (Cone(r, h)); origin at the centroid, apex at
+     * +3h/4 on y, base at -h/4; the support reads r, h/4, sin(half angle) */
+    double r = d[0], q = d[1] / 4.0;
+    o->volume = 4.0 * q * G_PI * r * r / 3.0;
+    double r2 = r * r, c = 0.2 * G_PI * q * r2;
+    I[0] = I[5] = c * (4.0 * q * q + r2);
+    I[3] = 2.0 * c * r2;
+    Iinv[5] = Iinv[0] = 1.0 / I[0];
+    Iinv[3] = 1.0 / I[3];
+    o->rc = fmax(sqrt(r * r + q * q), 3.0 * q);
+    /* Cone.cpp:116-122: the reference box is centred on the centroid and
+     * leaves out the apex region y in (h/2, 3h/4] */
+    o->bbox[0] = r; o->bbox[1] = 2.0 * q; o->bbox[2] = r;
+    break;
+  }
   case GRAINS_RECTANGLE: {
     double LX = d[0], LY = d[1];
     o->volume = 4.0 * LX * LY;
@@ -466,6 +505,11 @@ static int shape_derive(oshape* o, const grains_shape_t* s, const double* verts)
     return GRAINS_EUNSUPPORTED;
   }
   if (s->kind != GRAINS_SUPERQUADRIC) o->rc_exact = o->rc;
+  for (int k = 0; k < 3; ++k) o->bbox_exact[k] = o->bbox[k];
+  if (s->kind == GRAINS_CONE) o->bbox_exact[1] = 3.0 * d[1] / 4.0;
+  memcpy(o->gd, s->dims, sizeof(o->gd));
+  if (s->kind == GRAINS_CYLINDER) o->gd[1] = d[1] / 2.0;
+  if (s->kind == GRAINS_CONE) cone_dims(o->gd, d[0], d[1] / 4.0);
   o->mass = s->density * o->volume;
   if (s->density == 0) {
     for (int i = 0; i < 6; ++i) o->inertia[i] = o->inv[i] = 0;
@@ -478,9 +522,16 @@ static int shape_derive(oshape* o, const grains_shape_t* s, const double* verts)
   /* m_scaling (RigidBody.cpp:27-31) and the crust core it defines */
   for (int k = 0; k < 3; ++k)
     o->scaling[k] = o->bbox[k] > 0 ? (o->bbox[k] - s->crust) / o->bbox[k] : 1.0;
-  memcpy(o->core_dims, s->dims, sizeof(o->core_dims));
+  memcpy(o->core_dims, o->gd, sizeof(o->core_dims));
   switch (s->kind) {
   case GRAINS_SPHERE: o->core_dims[0] = d[0] - s->crust; break;
+  case GRAINS_CYLINDER: /* Cylinder(r - ct, h - 2 ct): the box scaling of m_scaling */
+    o->core_dims[0] = o->gd[0] - s->crust;
+    o->core_dims[1] = o->gd[1] - s->crust;
+    break;
+  case GRAINS_CONE: /* Cone(r - ct, h - 2 ct): y scaled by (h/2 - ct)/(h/2) about the centroid */
+    cone_dims(o->core_dims, o->gd[0] - s->crust, o->gd[1] - 0.5 * s->crust);
+    break;
   case GRAINS_BOX:
   case GRAINS_SUPERQUADRIC:
     for (int k = 0; k < 3; ++k) o->core_dims[k] = d[k] - s->crust;
@@ -519,10 +570,10 @@ static v3 sq_support(const double* d, v3 v) {
 
 /* Support of shape `o` (full shape when core == 0, crust core otherwise).
  * Sphere.cpp:111-114 (r * v/|v|, NaN at v = 0), Box.cpp:129-134,
- * Rectangle.cpp:115-120, Superquadric.cpp:169-205; polyhedron: first vertex
- * maximising v.x. */
+ * Rectangle.cpp:115-120, Superquadric.cpp:169-205, Cylinder.cpp:118-131,
+ * Cone.cpp:126-141; polyhedron: first vertex maximising v.x. */
 static v3 support(const oshape* o, v3 v, int core) {
-  const double* d = core ? o->core_dims : o->s.dims;
+  const double* d = core ? o->core_dims : o->gd;
   switch (o->s.kind) {
   case GRAINS_SPHERE: {
     double n = vnorm(v);
@@ -535,6 +586,24 @@ static v3 support(const oshape* o, v3 v, int core) {
     return sq_support(d, v);
   case GRAINS_RECTANGLE:
     return V(v.x < 0.0 ? -d[0] : d[0], v.y < 0.0 ? -d[1] : d[1], 0.0);
+  case GRAINS_CYLINDER: {
+    double s = sqrt(v.x * v.x + v.z * v.z);
+    double hy = v.y < 0.0 ? -d[1] : d[1];
+    if (s > EPS) {
+      double k = d[0] / s;
+      return V(v.x * k, hy, v.z * k);
+    }
+    return V(0.0, hy, 0.0);
+  }
+  case GRAINS_CONE: {
+    if (v.y > vnorm(v) * d[2]) return V(0.0, 3.0 * d[1], 0.0);
+    double s = sqrt(v.x * v.x + v.z * v.z);
+    if (s > EPS) {
+      double k = d[0] / s;
+      return V(v.x * k, -d[1], v.z * k);
+    }
+    return V(0.0, -d[1], 0.0);
+  }
   case GRAINS_POLYHEDRON: {
     const double* P = core ? o->core_verts : o->verts;
     int best = 0;
@@ -1072,7 +1141,8 @@ static cinfo closest_points(const oshape* A, const oshape* B, const tr3* a2w, co
   double overlap = vnorm(vsub(b2w->o, a2w->o)) -
                    (quirks ? A->rc + B->rc : A->rc_exact + B->rc_exact);
   if (overlap < 0.0) {
-    if (obb_intersect(A->bbox, B->bbox, a2w, b2w)) {
+    if (quirks ? obb_intersect(A->bbox, B->bbox, a2w, b2w)
+               : obb_intersect(A->bbox_exact, B->bbox_exact, a2w, b2w)) {
       v3 ptA, ptB;
       int it = 0, deep = 0, separated = 0;
       double ctSum = A->s.crust + B->s.crust;
@@ -1784,8 +1854,8 @@ double go_gjk(const grains_shape_t* a, const grains_shape_t* b, const double* ve
   oshape A, B;
   if (shape_derive(&A, a, vertices) || shape_derive(&B, b, vertices)) return NAN;
   /* distance of the full shapes: run with core supports equal to the shape */
-  memcpy(A.core_dims, A.s.dims, sizeof(A.core_dims));
-  memcpy(B.core_dims, B.s.dims, sizeof(B.core_dims));
+  memcpy(A.core_dims, A.gd, sizeof(A.core_dims));
+  memcpy(B.core_dims, B.gd, sizeof(B.core_dims));
   if (A.core_verts) memcpy(A.core_verts, A.verts, sizeof(double) * 3 * A.nv);
   if (B.core_verts) memcpy(B.core_verts, B.verts, sizeof(double) * 3 * B.nv);
   tr3 TA = tr_from12(ta), TB = tr_from12(tb);

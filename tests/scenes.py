@@ -128,3 +128,90 @@ def lattice_superquadrics(n_side=6, seed=9, gap=0.0, vel=0.02, omega=20.0, crust
                    vel=rng.standard_normal((n, 3)) * vel, omega=rng.standard_normal((n, 3)) * omega,
                    obstacle_shape=np.arange(n, n + len(ws), dtype=np.uint32), obstacle_tr=otr,
                    dt=dt, name=f"lattice_superquadrics_{n}")
+
+
+def dense_cylinders_cones(n=300, seed=41, phi=0.22, vel=0.05, omega=10.0, crust_frac=0.08, kr=0.0,
+                          shares=(0.4, 0.4, 0.2)):
+    """Random cylinders, cones and superquadrics (radius / semi-axes in
+    [0.3, 0.6] mm, heights in [0.6, 1.6] mm) at random positions and
+    orientations, periodic in x and y between a floor and a ceiling: many
+    crust-core overlaps of every pair type.  `kr`: rolling resistance."""
+    from grains3dgpu_amd.shapes import superquadric_volume
+    rng = np.random.default_rng(seed)
+    kind = rng.choice([A.CYLINDER, A.CONE, A.SUPERQUADRIC], size=n, p=shares)
+    shapes, vol = [], []
+    for k in kind:
+        if k == A.SUPERQUADRIC:
+            p = S.superquadric_params(rng, 1, a_rng=(0.3e-3, 0.6e-3))[0]
+            shapes.append(dict(kind=A.SUPERQUADRIC, material=0, dims=tuple(p), crust=crust_frac * p[:3].min(),
+                               density=2500.0))
+            vol.append(superquadric_volume(*p))
+        else:
+            r, h = rng.uniform(0.3e-3, 0.6e-3), rng.uniform(0.6e-3, 1.6e-3)
+            shapes.append(dict(kind=int(k), material=0, dims=(r, h), crust=crust_frac * min(r, h / 2),
+                               density=2500.0))
+            vol.append(math.pi * r * r * h / (1.0 if k == A.CYLINDER else 3.0))
+    L = (sum(vol) / phi) ** (1 / 3)
+    pos = rng.uniform(0, L, (n, 3))
+    pos[:, 2] = rng.uniform(1.0e-3, L - 1.0e-3, n)
+    first = len(shapes)
+    I = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0])
+    shapes += [dict(kind=A.RECTANGLE, dims=(L / 2, L / 2), density=0.0)] * 2
+    otr = np.array([np.concatenate([I, [L / 2, L / 2, 0]]), np.concatenate([I, [L / 2, L / 2, L]])])
+    return S.Scene(shapes=shapes, tables=np.array([S.hooke_row(1e5, 0.8, 2e4, 0.3, kr)]), n_materials=1,
+                   sims=[dict(domain_min=(0, 0, 0), domain_max=(L, L, L), gravity=(0, 0, -9.81),
+                              periodic=0b011)],
+                   particle_shape=np.arange(n, dtype=np.uint32), pos=pos,
+                   quat=S.random_quaternions(rng, n), vel=rng.standard_normal((n, 3)) * vel,
+                   omega=rng.standard_normal((n, 3)) * omega,
+                   obstacle_shape=np.array([first, first + 1], np.uint32), obstacle_tr=otr,
+                   dt=1e-6, name=f"dense_cylinders_cones_{n}")
+
+
+def lattice_cylinders_cones(n_side=6, n_layers=5, seed=47, tilt=0.05, vel=0.02, omega=20.0, crust_frac=0.1,
+                            r=0.45e-3, h=1.0e-3, dt=1e-7):
+    """Upright columns on a square lattice, cylinders and cones alternating
+    like a checkerboard (axis y turned onto z, then tilted by at most `tilt`
+    rad): neighbours' crusts overlap by a quarter crust at cylinder mantles,
+    cylinder caps, the cone bases' rims and each cone's apex pressed into the
+    base of the cone above; the bottom layer, lifted by r sin(tilt), sinks
+    at most 0.05 crust into the floor.
+    Periodic in x and y between a floor and a ceiling: contact-rich dynamics
+    (dt as for lattice_superquadrics)."""
+    rng = np.random.default_rng(seed)
+    ct = crust_frac * r
+    sx, sz = 2 * r - 0.25 * ct, h - 0.25 * ct
+    Lxy = n_side * sx
+    shapes = [dict(kind=A.CYLINDER, material=0, dims=(r, h), crust=ct, density=2500.0),
+              dict(kind=A.CONE, material=0, dims=(r, h), crust=ct, density=2500.0)]
+    kinds, pos = [], []
+    for k in range(n_layers):
+        for j in range(n_side):
+            for i in range(n_side):
+                cone = (i + j) % 2
+                z0 = (0.25 * h if cone else 0.5 * h) - 0.05 * ct + r * math.sin(tilt)
+                kinds.append(cone)
+                pos.append([(i + 0.5) * sx, (j + 0.5) * sx, z0 + k * sz])
+    n = len(kinds)
+    pos = np.array(pos)
+    Lz = (n_layers - 1) * sz + h + 2 * ct + r * math.sin(tilt)
+    # body y onto world z: +90 degrees about x, then a random tilt
+    s0, c0 = math.sin(math.pi / 4), math.cos(math.pi / 4)
+    axis = rng.standard_normal((n, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    ang = rng.uniform(0, tilt, n)
+    t = np.concatenate([axis * np.sin(ang / 2)[:, None], np.cos(ang / 2)[:, None]], 1)
+    q = np.stack([t[:, 3] * s0 + t[:, 0] * c0,
+                  t[:, 1] * c0 + t[:, 2] * s0,
+                  t[:, 2] * c0 - t[:, 1] * s0,
+                  t[:, 3] * c0 - t[:, 0] * s0], 1)
+    I = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1.0])
+    shapes += [dict(kind=A.RECTANGLE, dims=(Lxy / 2, Lxy / 2), density=0.0)] * 2
+    otr = np.array([np.concatenate([I, [Lxy / 2, Lxy / 2, 0]]), np.concatenate([I, [Lxy / 2, Lxy / 2, Lz]])])
+    return S.Scene(shapes=shapes, tables=np.array([S.hooke_row(1e5, 0.8, 2e4, 0.3)]), n_materials=1,
+                   sims=[dict(domain_min=(0, 0, 0), domain_max=(Lxy, Lxy, Lz), gravity=(0, 0, -9.81),
+                              periodic=0b011)],
+                   particle_shape=np.array(kinds, np.uint32), pos=pos, quat=q,
+                   vel=rng.standard_normal((n, 3)) * vel, omega=rng.standard_normal((n, 3)) * omega,
+                   obstacle_shape=np.array([2, 3], np.uint32), obstacle_tr=otr,
+                   dt=dt, name=f"lattice_cylinders_cones_{n}")

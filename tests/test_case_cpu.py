@@ -199,3 +199,15 @@ def test_run_case_through_the_oracle_writes_rawdata(deck):
     assert p[0, 2] > POS[0][2]
     assert open(os.path.join(out, "Results_particleType.dat")).read() == "\n"
     assert open(os.path.join(out, "Results_coordinationNumber.dat")).read() == "\n" * 5
+
+
+def test_cylinder_and_cone_elements(tmp_path):
+    """<Cylinder Radius Height/> and <Cone Radius Height/> (Cylinder.cpp:26-30,
+    Cone.cpp:27-35) keep the full height in dims[1]."""
+    deck = DECK.replace('<Box LX="0.001" LY="0.0008" LZ="0.0006"/>', '<Cylinder Radius="4e-4" Height="1.2e-3"/>')
+    deck = deck.replace('<Superquadric a="6e-4" b="5e-4" c="4e-4" n1="4" n2="3"/>', '<Cone Radius="5e-4" Height="9e-4"/>')
+    (tmp_path / "d.xml").write_text(deck)
+    sh = C.read_case(str(tmp_path / "d.xml")).scene.shapes
+    assert sh["kind"].tolist() == [A.SPHERE, A.CYLINDER, A.CONE, A.RECTANGLE]
+    np.testing.assert_array_equal(sh["dims"][1][:2], [4e-4, 1.2e-3])
+    np.testing.assert_array_equal(sh["dims"][2][:2], [5e-4, 9e-4])
