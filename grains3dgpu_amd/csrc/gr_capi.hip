@@ -624,9 +624,11 @@ static int finalize(grains_ctx* ctx) {
   DevScene& d = ctx->d;
   const bool quirks = (ctx->cfg.flags & GRAINS_REF_QUIRKS) != 0;
   const uint32_t ns = (uint32_t)ctx->sims.size();
-  for (uint32_t s : ctx->obs_shape)
-    if (s >= ctx->shapes.size() || ctx->shapes[s].s.kind != GRAINS_RECTANGLE)
-      return fail(ctx, GRAINS_EUNSUPPORTED, "obstacles must be Rectangle walls");
+  d.gen_obs = 0;
+  for (uint32_t s : ctx->obs_shape) {
+    if (s >= ctx->shapes.size()) return fail(ctx, GRAINS_EINVAL, "obstacle shape index out of range");
+    if (ctx->shapes[s].s.kind != GRAINS_RECTANGLE) d.gen_obs = 1;
+  }
   for (uint32_t s : ctx->obs_sim)
     if (s >= ns) return fail(ctx, GRAINS_EINVAL, "obstacle sim index out of range");
   for (const auto& h : ctx->shapes)
@@ -1385,6 +1387,7 @@ extern "C" int grains_adjoint_record(grains_ctx_t* ctx, uint32_t checkpoint_ever
   }
   for (const auto& t : ctx->tables)
     if (t.kr != 0.0) return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers kr = 0 (no rolling resistance)");
+  if (ctx->d.gen_obs) return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers Rectangle obstacles");
   // the sweep's buffers are sized by checkpoint_every: drop them
   free_list(ctx->adj_cp);
   free_list(ctx->adj_allocs);

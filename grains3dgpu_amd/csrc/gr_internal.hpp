@@ -141,6 +141,7 @@ struct DevScene {
   uint32_t defer_cap;
   int32_t split;                // 1: screening pass + pass over the queued entries
   int32_t cand_staged;          // 1: candidate search stages neighbour windows in LDS
+  int32_t gen_obs;              // 1: some obstacle is not a Rectangle (GJK in the integration kernel)
   // per particle outputs
   int32_t* contact_count;
   double *last_force, *last_torque;

@@ -168,13 +168,47 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
 template <bool SV>
 __global__ void __launch_bounds__(kBlock) k_refine64(DevScene d);
 
-// Per particle: sum its entries in visit order, walls, gravity, then
+// Particle x obstacle of any other kind than Rectangle: closestPointsRigidBodies
+// (bounding spheres, OBB, GJK) against the obstacle at rest, with its zero
+// kinematics and mass (ComponentManagerCPU.cpp:301-351), in FP64 for both
+// precisions (obstacle contacts are few; no FP32 re-run is needed).
+__device__ __forceinline__ void obstacle_contact64(const DevScene& d, uint32_t si, uint32_t os, const double* t,
+                                                   uint32_t table, V3<double> xi, Qt<double> qi, V3<double> vel,
+                                                   V3<double> om, V3<double>& F, V3<double>& M, uint32_t& walls,
+                                                   NpStats& st) {
+  const ShapeNP<double> A = d.np64[si];
+  const ShapeNP<double> B = d.np64[os];
+  Tr<double> a2w, b2w;
+  a2w.R = rot(qi);
+  a2w.o = xi;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) b2w.R.m[k] = t[k];
+  b2w.o = {t[9], t[10], t[11]};
+  const GjkArgs args = {d.flags, d.gjk_rel};
+  const bool quirks = (d.flags & GRAINS_REF_QUIRKS) != 0;
+  const CInfo<double> ci =
+      quirks ? closest_points<double, false>(A, B, a2w, b2w, args, d.verts64, d.cverts64, st)
+             : closest_points<double, true>(A, B, a2w, b2w, args, d.verts64, d.cverts64, st);
+  if (ci.overlap < 0.0) {
+    const HookeP<double> h = d.hooke64[table + material_hash((uint32_t)A.material, (uint32_t)B.material)];
+    const V3<double> lever = quirks ? ci.cp : ci.cp - xi;
+    const V3<double> relVel = vel + cross(om, lever);
+    V3<double> f, m;
+    hooke<double>(h, ci, relVel, om, A.mass, B.mass, ci.cp - xi, f, m);
+    F = F + f;
+    M = M + m;
+    ++walls;
+  }
+}
+
+// Per particle: sum its entries in visit order, obstacles, gravity, then
 // moveParticles (ComponentManagerCPU.cpp:428-500) on the quaternion state.
-template <class T>
+// GEN: some obstacle is not a Rectangle (obstacle_contact64 for those).
+template <class T, bool GEN>
 __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
   const uint64_t n = d.n;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t walls = 0, contacts = 0;
+  uint32_t walls = 0, contacts = 0, c_gjk = 0, c_it = 0, c_deep = 0;
   if (i < n) {
     const PairRes<T>* res = (const PairRes<T>*)d.pair_res;
     const uint32_t s = d.rank[i];
@@ -199,8 +233,19 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
     if (g.obs_end > g.obs_begin) {
       const ShapeNP<T> A = np_table<T>(d)[si];
       for (uint32_t o = g.obs_begin; o < g.obs_end; ++o) {
-        const ShapeNP<T> B = np_table<T>(d)[d.obs_shape[o]];
         const double* t = d.obs_tr + 12 * (size_t)o;
+        if constexpr (GEN) {
+          const uint32_t os = d.obs_shape[o];
+          if (d.np64[os].kind != GRAINS_RECTANGLE) {
+            NpStats st = {0, 0, 0, 0, 0};
+            obstacle_contact64(d, si, os, t, g.table, xi, qi, vel, om, F, M, walls, st);
+            c_gjk += st.gjk;
+            c_it += st.iters;
+            c_deep += st.deep;
+            continue;
+          }
+        }
+        const ShapeNP<T> B = np_table<T>(d)[d.obs_shape[o]];
         Tr<T> a2w, b2w;
         M3<double> Ro;
 #pragma unroll
@@ -217,8 +262,8 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
           b2w.R = cvt<float>(Ro);
           b2w.o = cvt<float>(xo - xi);
         }
-        // obstacles are Rectangle walls (checked on the host): the plane branch
-        // of closestPointsRigidBodies (CollisionDetection.cpp:203-206)
+        // a Rectangle wall: the plane branch of closestPointsRigidBodies
+        // (CollisionDetection.cpp:203-206)
         CInfo<T> ci = cp_rectangle(sup_ctx(A, false, vt_full<T>(d), vt_core<T>(d)), a2w, b2w);
         if (ci.overlap < T(0)) {
           const HookeP<T> h = hk_table<T>(d)[g.table + material_hash((uint32_t)A.material, (uint32_t)B.material)];
@@ -272,6 +317,11 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
   count_add(d.counters, kCntWall, walls);
   // contacts are counted after any FP64 re-run has settled each entry
   count_add(d.counters, kCntContacts, contacts);
+  if constexpr (GEN) {
+    count_add(d.counters, kCntGjk, c_gjk);
+    count_add(d.counters, kCntGjkIters, c_it);
+    count_add(d.counters, kCntDeep, c_deep);
+  }
 }
 
 // GrainsTestDev workload: GJK distance of independent pairs of full shapes.

@@ -43,7 +43,11 @@ void launch_refine64(const DevScene& d, hipStream_t s) {
     hipLaunchKernelGGL(k_refine64<true>, dim3(d.narrow_blocks / 4 + 1), dim3(kBlock), 0, s, d);
 }
 void launch_integrate64(const DevScene& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_integrate<double>, dim3((unsigned)((d.n + d.block - 1) / d.block)), dim3(d.block), 0, s, d);
+  const dim3 grid((unsigned)((d.n + d.block - 1) / d.block));
+  if (d.gen_obs)
+    hipLaunchKernelGGL((k_integrate<double, true>), grid, dim3(d.block), 0, s, d);
+  else
+    hipLaunchKernelGGL((k_integrate<double, false>), grid, dim3(d.block), 0, s, d);
 }
 void launch_gjk_batch64(const ShapeNP<double>* shapes, const double* verts, const uint32_t* sa,
                         const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,

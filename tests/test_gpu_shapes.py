@@ -196,3 +196,62 @@ def test_gjk_batch_cylinders_cones():
         assert np.isfinite(d).all()
         assert rel_err(d, ref) < tol, (prec, flags, rel_err(d, ref))
         e.close()
+
+
+def with_obstacles(sc):
+    """The scene plus three obstacles at rest between its walls: a tilted Box
+    at the centre, a Cylinder lying along x and a Superquadric, all of
+    density 0 (mass 0: the avmass branch of HookeContactForceModel.cpp:124-129)."""
+    L = sc.sims[0]["domain_max"][0]
+    shapes = list(sc.shapes)
+    first = len(shapes)
+    shapes += [dict(kind=A.BOX, material=0, dims=(L / 8, L / 10, L / 12), crust=1e-5, density=0.0),
+               dict(kind=A.CYLINDER, material=0, dims=(L / 12, 0.9 * L), crust=1e-5, density=0.0),
+               dict(kind=A.SUPERQUADRIC, material=0, dims=(L / 10, L / 9, L / 8, 4.0, 3.0), crust=1e-5, density=0.0)]
+    q = np.array([[0.1, 0.2, 0.05, 1.0], [0.0, 0.0, np.sin(np.pi / 4), np.cos(np.pi / 4)], [0.3, -0.1, 0.2, 0.9]])
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    pos = np.array([[L / 2, L / 2, L / 2], [L / 2, L / 4, 0.3 * L], [0.25 * L, 0.75 * L, 0.7 * L]])
+    otr = np.concatenate([sc.obstacle_tr, S.transform12(pos, q)])
+    osh = np.concatenate([sc.obstacle_shape, np.arange(first, first + 3, dtype=np.uint32)])
+    return dataclasses.replace(sc, shapes=shapes, obstacle_shape=osh, obstacle_tr=otr)
+
+
+@pytest.mark.parametrize("flags", [0, A.REF_QUIRKS])
+def test_general_obstacles_fp64(flags):
+    """Obstacles of any convex kind go through the bounding-sphere, OBB and
+    GJK tests against every particle of their simulation
+    (ComponentManagerCPU.cpp:301-351).  Wall counts and GJK calls equal the
+    oracle's; forces to the bars of the first-step test."""
+    sc = with_obstacles(dense_cylinders_cones(n=300, seed=48))
+    e, o = both(sc, flags=flags, tol=1e-12)
+    e.step(1)
+    o.step(1)
+    se, so = e.get_stats(), o.get_stats()
+    # (the reference's Johnson GJK stops at once on overlapping shapes, so
+    # under REF_QUIRKS no obstacle contact of this scene is found)
+    assert se["wall_contacts"] == so["wall_contacts"] and so["wall_contacts"] >= (0 if flags else 20)
+    assert se["gjk_calls"] == so["gjk_calls"]
+    np.testing.assert_array_equal(e.get_contact_count(), o.get_contact_count())
+    fe, te = e.get_torce()
+    fo, to = o.get_torce()
+    bar = 1e-9 if flags & A.REF_QUIRKS else 10.0 * np.sqrt(2e-12)
+    assert rel_err(fe, fo) < bar and rel_err(te, to) < bar
+    # the obstacles' share: forces without them differ
+    from oracle import pyoracle
+    o0 = S.load(pyoracle.Oracle(dt=sc.dt, flags=flags, gjk_rel_tol=1e-12),
+                dataclasses.replace(sc, obstacle_shape=sc.obstacle_shape[:2], obstacle_tr=sc.obstacle_tr[:2]))
+    o0.step(1)
+    if not flags:
+        assert rel_err(o0.get_torce()[0], fo) > 1e-3
+
+
+def test_general_obstacles_fp32_mixed():
+    sc = with_obstacles(dense_cylinders_cones(n=300, seed=49))
+    e, o = both(sc, precision=A.FP32_MIXED, tol=1e-6, oracle_tol=1e-12)
+    e.step(1)
+    o.step(1)
+    se, so = e.get_stats(), o.get_stats()
+    assert abs(int(se["wall_contacts"]) - int(so["wall_contacts"])) <= 2
+    fe, _ = e.get_torce()
+    fo, _ = o.get_torce()
+    assert rel_err(fe, fo) < np.sqrt(2e-6)
