@@ -32,7 +32,7 @@ with the RawData writer.
 - Simulation: ParticleInsertion (InitialPosition, InitialOrientation,
   InitialVelocity, InitialAngularVelocity, each Type Zero | Constant X Y Z |
   File Name | Random with Box windows; Insertion.cpp), PostProcessing
-  TimeSave Start/End/dt and Writers RawData Directory/RootName.
+  TimeSave Start/End/dt and Writers RawData / Paraview Directory/RootName.
 
 Differences from the reference, each where the reference is undefined or
 cannot run:
@@ -312,10 +312,10 @@ def read_case(path, base_dir=None):
         for w in (list(ws) if ws is not None else []):
             if not isinstance(w.tag, str):
                 continue
-            if w.tag != "RawData":
-                raise CaseError(f"writer {w.tag} is not supported (RawData)")
+            if w.tag not in ("RawData", "Paraview"):
+                raise CaseError(f"writer {w.tag} is not supported (RawData, Paraview)")
             d = w.get("Directory", ".")
-            writers.append(Writer("RawData", d if os.path.isabs(d) else os.path.join(base_dir, d),
+            writers.append(Writer(w.tag, d if os.path.isabs(d) else os.path.join(base_dir, d),
                                   w.get("RootName", "Results")))
     n = len(particle_shape)
     sc = Scene(shapes=np.array(shapes, SHAPE_DTYPE), tables=tables, n_materials=nm,
@@ -438,9 +438,10 @@ def run_case(case, backend, log=None):
     or any object with the grains_set_* / step / get_state calls), firing
     the writers at the save times.  Returns the number of steps taken."""
     from .scene import load
+    from .paraview import ParaviewWriter
     sc = insert(case)
     load(backend, sc)
-    writers = [RawDataWriter(w.directory, w.root) for w in case.writers]
+    writers = [(RawDataWriter if w.kind == "RawData" else ParaviewWriter)(w.directory, w.root) for w in case.writers]
     for w in writers:
         w.start()
     T = np.float32 if case.precision == A.FP32_MIXED else np.float64
@@ -455,9 +456,12 @@ def run_case(case, backend, log=None):
             steps += pending
             pending = 0
             saves.pop(0)
-            p, _, v, w = backend.get_state()
+            p, q, v, w = backend.get_state()
             for wr in writers:
-                wr.write(float(t), p, v, w, obs_pos)
+                if isinstance(wr, RawDataWriter):
+                    wr.write(float(t), p, v, w, obs_pos)
+                else:
+                    wr.write(float(t), sc, p, q, v, w)
             if log:
                 log(f"t = {float(t):.6e}: {steps} steps")
         t = T(t + dt)

@@ -211,3 +211,29 @@ def test_cylinder_and_cone_elements(tmp_path):
     assert sh["kind"].tolist() == [A.SPHERE, A.CYLINDER, A.CONE, A.RECTANGLE]
     np.testing.assert_array_equal(sh["dims"][1][:2], [4e-4, 1.2e-3])
     np.testing.assert_array_equal(sh["dims"][2][:2], [5e-4, 9e-4])
+
+
+def test_run_case_writes_paraview(deck):
+    """A Paraview writer beside RawData: one .vtu per save for particles and
+    obstacles, and collections that list every save with its time."""
+    from oracle import pyoracle
+    tmp_path = deck.parent
+    deck = DECK.replace('<Writers><RawData Directory="out" RootName="Results"/></Writers>',
+                        '<Writers><RawData Directory="out" RootName="Results"/>'
+                        '<Paraview Directory="pv" RootName="run"/></Writers>')
+    (tmp_path / "d.xml").write_text(deck)
+    c = C.read_case(str(tmp_path / "d.xml"))
+    assert [w.kind for w in c.writers] == ["RawData", "Paraview"]
+    o = pyoracle.Oracle(dt=c.dt, precision=c.precision)
+    C.run_case(c, o)
+    pv = tmp_path / "pv"
+    names = sorted(os.listdir(pv))
+    assert names == sorted([f"run_{p}_T{k}.vtu" for p in ("Particles", "Obstacles") for k in range(5)]
+                           + ["run_Particles.pvd", "run_Obstacles.pvd"])
+    import xml.etree.ElementTree as ET
+    ds = ET.parse(pv / "run_Particles.pvd").getroot().find("Collection").findall("DataSet")
+    assert [d.get("timestep") for d in ds] == ["0", "5e-06", "1e-05", "1.5e-05", "2e-05"]
+    piece = ET.parse(pv / "run_Particles_T4.vtu").getroot().find("UnstructuredGrid").find("Piece")
+    # 3 spheres, 2 boxes, 1 superquadric
+    assert int(piece.get("NumberOfPoints")) == 3 * 483 + 2 * 8 + 243
+    assert int(piece.get("NumberOfCells")) == 3 * 512 + 2 * 1 + 256
