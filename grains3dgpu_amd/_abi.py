@@ -15,6 +15,9 @@ FP64, FP32_MIXED = 0, 1
 REF_QUIRKS = 0x1
 NESTEROV = 0x2
 
+# grains_config_t.integrator
+FIRST_ORDER_EXPLICIT, SECOND_ORDER_EXPLICIT, SECOND_ORDER_LEAPFROG, SECOND_ORDER_ADAMS_BASHFORTH = range(4)
+
 PAIRS_CANDIDATE, PAIRS_CONTACT = 0, 1
 
 
@@ -42,7 +45,7 @@ class Config(C.Structure):
                 ("flags", C.c_uint32), ("gjk_rel_tol", C.c_double),
                 ("max_neighbors", C.c_int32), ("block_size", C.c_int32),
                 ("narrow_blocks_per_cu", C.c_int32), ("narrow_passes", C.c_int32),
-                ("candidate_staging", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("candidate_staging", C.c_int32), ("integrator", C.c_int32)]
 
 
 class Stats(C.Structure):

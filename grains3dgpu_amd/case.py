@@ -27,7 +27,9 @@ with the RawData writer.
   (ContactForceModelBuilderFactory.cpp:88-121, HookeContactForceModel.cpp:
   15-45), indexed by computeHash;
 - TemporalSetting: TimeInterval Start, End, dt; TimeIntegration
-  FirstOrderExplicit;
+  FirstOrderExplicit, or one of the three second-order schemes the
+  reference names but does not build (grains_hip.h states their updates;
+  create the backend with integrator=case.integrator);
 - Forces: Gravity GX, GY, GZ;
 - Simulation: ParticleInsertion (InitialPosition, InitialOrientation,
   InitialVelocity, InitialAngularVelocity, each Type Zero | Constant X Y Z |
@@ -71,6 +73,13 @@ from .scene import Scene, SHAPE_DTYPE
 HIGHEPS = {A.FP64: 1e-15, A.FP32_MIXED: 1e-8}  # Basic.hh:116-118
 
 
+# TimeIntegration Type values (TimeIntegratorBuilderFactory.cpp:39-49; the
+# reference builds only FirstOrderExplicit and exits on the other three)
+INTEGRATORS = {"FirstOrderExplicit": A.FIRST_ORDER_EXPLICIT, "SecondOrderExplicit": A.SECOND_ORDER_EXPLICIT,
+               "SecondOrderLeapFrog": A.SECOND_ORDER_LEAPFROG,
+               "SecondOrderAdamsBashforth": A.SECOND_ORDER_ADAMS_BASHFORTH}
+
+
 class CaseError(ValueError):
     pass
 
@@ -94,6 +103,7 @@ class Case:
     materials: dict
     insertion: dict = field(default_factory=dict)
     particle_tr: np.ndarray | None = None  # per particle: the Particle node's transform (12)
+    integrator: int = A.FIRST_ORDER_EXPLICIT  # grains_config_t.integrator of the TimeIntegration Type
 
 
 def _f(node, name, default=None):
@@ -281,8 +291,9 @@ def read_case(path, base_dir=None):
     ts = _child(cons, "TemporalSetting", True)
     ti = _child(ts, "TimeInterval", True)
     integ = ts.find("TimeIntegration")
-    if integ is not None and integ.get("Type", "FirstOrderExplicit") != "FirstOrderExplicit":
-        raise CaseError(f"time integration {integ.get('Type')} is not supported (FirstOrderExplicit)")
+    scheme = "FirstOrderExplicit" if integ is None else integ.get("Type", "FirstOrderExplicit")
+    if scheme not in INTEGRATORS:
+        raise CaseError(f"time integration {scheme} is not known ({', '.join(INTEGRATORS)})")
     g = [0.0, 0.0, 0.0]
     forces = root.find("Forces")
     if forces is not None:
@@ -328,7 +339,7 @@ def read_case(path, base_dir=None):
                name=os.path.basename(path))
     return Case(scene=sc, precision=prec, t_start=_f(ti, "Start"), t_end=_f(ti, "End"), dt=_f(ti, "dt"),
                 save_times=save, writers=writers, materials=materials, insertion=insertion,
-                particle_tr=np.array(ptr).reshape(-1, 12))
+                particle_tr=np.array(ptr).reshape(-1, 12), integrator=INTEGRATORS[scheme])
 
 
 def _draw(rule, n):

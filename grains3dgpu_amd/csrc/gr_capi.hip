@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/grains_hip.h"
@@ -462,6 +463,10 @@ extern "C" int grains_create(const grains_config_t* cfg, grains_ctx_t** out) {
     delete ctx;
     return GRAINS_EINVAL;
   }
+  if (ctx->cfg.integrator < GRAINS_FIRST_ORDER_EXPLICIT || ctx->cfg.integrator > GRAINS_SECOND_ORDER_ADAMS_BASHFORTH) {
+    delete ctx;
+    return GRAINS_EINVAL;
+  }
   std::memset(&ctx->stats, 0, sizeof(ctx->stats));
   std::memset(&ctx->d, 0, sizeof(ctx->d));
   int ndev = 0;
@@ -541,12 +546,26 @@ extern "C" int grains_set_obstacles(grains_ctx_t* ctx, const uint32_t* shape, co
   return GRAINS_OK;
 }
 
+// The schemes' history: NaN in every plane, so the next step is a first-order
+// one (ADAMS_BASHFORTH) or a half kick (LEAPFROG).
+static int hist_reset(grains_ctx* ctx) {
+  if (!ctx->d.ab_hist) return GRAINS_OK;
+  std::vector<double> nan(12 * ctx->n, std::numeric_limits<double>::quiet_NaN());
+  GR_HIP(hipMemcpy(ctx->d.ab_hist, nan.data(), nan.size() * sizeof(double), hipMemcpyHostToDevice));
+  return GRAINS_OK;
+}
+
 static int alloc_state(grains_ctx* ctx) {
   free_list(ctx->state_allocs);
   const uint64_t n = ctx->n;
   DevScene& d = ctx->d;
   auto& L = ctx->state_allocs;
   int rc;
+  d.ab_hist = nullptr;
+  if ((ctx->cfg.integrator == GRAINS_SECOND_ORDER_ADAMS_BASHFORTH ||
+       ctx->cfg.integrator == GRAINS_SECOND_ORDER_LEAPFROG) &&
+      ((rc = dalloc(ctx, &d.ab_hist, 12 * n * sizeof(double), L)) || (rc = hist_reset(ctx))))
+    return rc;
   if ((rc = dalloc(ctx, &d.pos, 3 * n * sizeof(double), L)) || (rc = dalloc(ctx, &d.quat, 4 * n * sizeof(double), L)) ||
       (rc = dalloc(ctx, &d.vel, 3 * n * sizeof(double), L)) || (rc = dalloc(ctx, &d.omega, 3 * n * sizeof(double), L)) ||
       (rc = dalloc(ctx, &d.shape, n * sizeof(uint32_t), L)) || (rc = dalloc(ctx, &d.sim, n * sizeof(uint32_t), L)) ||
@@ -805,6 +824,7 @@ static int finalize(grains_ctx* ctx) {
   d.flags = ctx->cfg.flags;
   d.gjk_rel = ctx->cfg.gjk_rel_tol;
   d.precision = ctx->cfg.precision;
+  d.integrator = ctx->cfg.integrator;
   d.block = (uint32_t)ctx->cfg.block_size;
   d.narrow_blocks = (uint32_t)(cu_count() * ctx->cfg.narrow_blocks_per_cu);
   d.split = ctx->cfg.narrow_passes == 2 ? 1 : 0;
@@ -837,7 +857,7 @@ extern "C" int grains_set_state(grains_ctx_t* ctx, const double* pos, const doub
   if ((rc = put(pos, ctx->d.pos, 3)) || (rc = put(quat, ctx->d.quat, 4)) || (rc = put(vel, ctx->d.vel, 3)) ||
       (rc = put(omega, ctx->d.omega, 3)))
     return rc;
-  return GRAINS_OK;
+  return hist_reset(ctx);
 }
 
 extern "C" int grains_get_state(grains_ctx_t* ctx, double* pos, double* quat, double* vel, double* omega,
@@ -1328,7 +1348,8 @@ extern "C" int grains_gjk_batch(grains_ctx_t* ctx, const uint32_t* sa, const uin
 extern "C" int grains_snapshot(grains_ctx_t* ctx, void* bytes, size_t* size) {
   if (!ctx || !size) return GRAINS_EINVAL;
   const uint64_t n = ctx->n;
-  const size_t need = 8 + 13 * n * 8 + n * 4;
+  const size_t nh = ctx->d.ab_hist ? 12 * n * 8 : 0;  // the two-step scheme's history
+  const size_t need = 8 + 13 * n * 8 + n * 4 + nh;
   if (!bytes) {
     *size = need;
     return GRAINS_OK;
@@ -1347,6 +1368,8 @@ extern "C" int grains_snapshot(grains_ctx_t* ctx, void* bytes, size_t* size) {
   GR_HIP(hipMemcpy(p, ctx->d.omega, 3 * n * 8, hipMemcpyDeviceToHost));
   p += 3 * n * 8;
   GR_HIP(hipMemcpy(p, ctx->d.contact_count, n * 4, hipMemcpyDeviceToHost));
+  p += n * 4;
+  if (nh) GR_HIP(hipMemcpy(p, ctx->d.ab_hist, nh, hipMemcpyDeviceToHost));
   *size = need;
   return GRAINS_OK;
 }
@@ -1357,7 +1380,8 @@ extern "C" int grains_restore(grains_ctx_t* ctx, const void* bytes, size_t size)
   uint64_t n0;
   if (size < 8) return GRAINS_EINVAL;
   std::memcpy(&n0, bytes, 8);
-  if (n0 != n || size < 8 + 13 * n * 8 + n * 4) return fail(ctx, GRAINS_EINVAL, "snapshot does not match");
+  const size_t nh = ctx->d.ab_hist ? 12 * n * 8 : 0;
+  if (n0 != n || size < 8 + 13 * n * 8 + n * 4 + nh) return fail(ctx, GRAINS_EINVAL, "snapshot does not match");
   GR_HIP(hipStreamSynchronize(ctx->stream));
   const char* p = (const char*)bytes + 8;
   GR_HIP(hipMemcpy(ctx->d.pos, p, 3 * n * 8, hipMemcpyHostToDevice));
@@ -1369,6 +1393,8 @@ extern "C" int grains_restore(grains_ctx_t* ctx, const void* bytes, size_t size)
   GR_HIP(hipMemcpy(ctx->d.omega, p, 3 * n * 8, hipMemcpyHostToDevice));
   p += 3 * n * 8;
   GR_HIP(hipMemcpy(ctx->d.contact_count, p, n * 4, hipMemcpyHostToDevice));
+  p += n * 4;
+  if (nh) GR_HIP(hipMemcpy(ctx->d.ab_hist, p, nh, hipMemcpyHostToDevice));
   return GRAINS_OK;
 }
 
@@ -1388,6 +1414,8 @@ extern "C" int grains_adjoint_record(grains_ctx_t* ctx, uint32_t checkpoint_ever
   for (const auto& t : ctx->tables)
     if (t.kr != 0.0) return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers kr = 0 (no rolling resistance)");
   if (ctx->d.gen_obs) return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers Rectangle obstacles");
+  if (ctx->cfg.integrator != GRAINS_FIRST_ORDER_EXPLICIT)
+    return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers FirstOrderExplicit");
   // the sweep's buffers are sized by checkpoint_every: drop them
   free_list(ctx->adj_cp);
   free_list(ctx->adj_allocs);

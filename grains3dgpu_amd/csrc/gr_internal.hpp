@@ -142,6 +142,11 @@ struct DevScene {
   int32_t split;                // 1: screening pass + pass over the queued entries
   int32_t cand_staged;          // 1: candidate search stages neighbour windows in LDS
   int32_t gen_obs;              // 1: some obstacle is not a Rectangle (GJK in the integration kernel)
+  // time integration: scheme, and its history (12 planes of n): for
+  // ADAMS_BASHFORTH the previous step's v, w, a, alpha (NaN: none yet), for
+  // LEAPFROG plane 0 NaN until the first (half) kick
+  int32_t integrator;
+  double* ab_hist;
   // per particle outputs
   int32_t* contact_count;
   double *last_force, *last_torque;

@@ -204,7 +204,8 @@ __device__ __forceinline__ void obstacle_contact64(const DevScene& d, uint32_t s
 // Per particle: sum its entries in visit order, obstacles, gravity, then
 // moveParticles (ComponentManagerCPU.cpp:428-500) on the quaternion state.
 // GEN: some obstacle is not a Rectangle (obstacle_contact64 for those).
-template <class T, bool GEN>
+// SO: a second-order scheme (d.integrator); otherwise FirstOrderExplicit.
+template <class T, bool GEN, bool SO>
 __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
   const uint64_t n = d.n;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,7 +296,17 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
     contacts = (uint32_t)ncon;
     V3<double> x = xi, vnew = vel, onew = om;
     Qt<double> qn = qi;
-    move_particle(dy.mass, dy.inertia, dy.inv, d.dt, F, M, x, qn, vnew, onew);
+    if constexpr (!SO) {
+      move_particle(dy.mass, dy.inertia, dy.inv, d.dt, F, M, x, qn, vnew, onew);
+    } else {
+      V3<double> hist[4];
+      const int nh = d.integrator == GRAINS_SECOND_ORDER_ADAMS_BASHFORTH ? 4
+                     : d.integrator == GRAINS_SECOND_ORDER_LEAPFROG    ? 1
+                                                                       : 0;
+      for (int k = 0; k < nh; ++k) hist[k] = load3(d.ab_hist + 3 * k * n, n, i);
+      move_particle_scheme(d.integrator, dy.mass, dy.inertia, dy.inv, d.dt, F, M, x, qn, vnew, onew, hist);
+      for (int k = 0; k < nh; ++k) store3(d.ab_hist + 3 * k * n, n, i, hist[k]);
+    }
     if (g.periodic) {
       // periodic wrap of the centre (not in the reference)
       auto wrap = [&](double c, int k) {

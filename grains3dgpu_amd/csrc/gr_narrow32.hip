@@ -21,10 +21,15 @@ void launch_narrow32(const DevScene& d, int pass, hipStream_t s) {
 }
 void launch_integrate32(const DevScene& d, hipStream_t s) {
   const dim3 grid((unsigned)((d.n + d.block - 1) / d.block));
-  if (d.gen_obs)
-    hipLaunchKernelGGL((k_integrate<float, true>), grid, dim3(d.block), 0, s, d);
+  const bool so = d.integrator != GRAINS_FIRST_ORDER_EXPLICIT;
+  if (d.gen_obs && so)
+    hipLaunchKernelGGL((k_integrate<float, true, true>), grid, dim3(d.block), 0, s, d);
+  else if (d.gen_obs)
+    hipLaunchKernelGGL((k_integrate<float, true, false>), grid, dim3(d.block), 0, s, d);
+  else if (so)
+    hipLaunchKernelGGL((k_integrate<float, false, true>), grid, dim3(d.block), 0, s, d);
   else
-    hipLaunchKernelGGL((k_integrate<float, false>), grid, dim3(d.block), 0, s, d);
+    hipLaunchKernelGGL((k_integrate<float, false, false>), grid, dim3(d.block), 0, s, d);
 }
 void launch_gjk_batch32(const ShapeNP<float>* shapes, const float* verts, const uint32_t* sa,
                         const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,

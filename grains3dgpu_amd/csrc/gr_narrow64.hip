@@ -44,10 +44,15 @@ void launch_refine64(const DevScene& d, hipStream_t s) {
 }
 void launch_integrate64(const DevScene& d, hipStream_t s) {
   const dim3 grid((unsigned)((d.n + d.block - 1) / d.block));
-  if (d.gen_obs)
-    hipLaunchKernelGGL((k_integrate<double, true>), grid, dim3(d.block), 0, s, d);
+  const bool so = d.integrator != GRAINS_FIRST_ORDER_EXPLICIT;
+  if (d.gen_obs && so)
+    hipLaunchKernelGGL((k_integrate<double, true, true>), grid, dim3(d.block), 0, s, d);
+  else if (d.gen_obs)
+    hipLaunchKernelGGL((k_integrate<double, true, false>), grid, dim3(d.block), 0, s, d);
+  else if (so)
+    hipLaunchKernelGGL((k_integrate<double, false, true>), grid, dim3(d.block), 0, s, d);
   else
-    hipLaunchKernelGGL((k_integrate<double, false>), grid, dim3(d.block), 0, s, d);
+    hipLaunchKernelGGL((k_integrate<double, false, false>), grid, dim3(d.block), 0, s, d);
 }
 void launch_gjk_batch64(const ShapeNP<double>* shapes, const double* verts, const uint32_t* sa,
                         const uint32_t* sb, const double* ta, const double* tb, uint64_t n, GjkArgs args,

@@ -75,7 +75,8 @@ class Engine:
     """One grains_ctx_t.  precision: _abi.FP64 | _abi.FP32_MIXED."""
 
     def __init__(self, dt=1e-6, precision=A.FP64, flags=0, gjk_rel_tol=0.0, max_neighbors=0,
-                 block_size=0, narrow_blocks_per_cu=0, narrow_passes=0, candidate_staging=0):
+                 block_size=0, narrow_blocks_per_cu=0, narrow_passes=0, candidate_staging=0,
+                 integrator=A.FIRST_ORDER_EXPLICIT):
         self.L = lib()
         cfg = A.Config()
         cfg.dt, cfg.precision, cfg.flags, cfg.gjk_rel_tol = dt, precision, flags, gjk_rel_tol
@@ -83,6 +84,7 @@ class Engine:
         cfg.narrow_blocks_per_cu = narrow_blocks_per_cu
         cfg.narrow_passes = narrow_passes
         cfg.candidate_staging = candidate_staging
+        cfg.integrator = integrator
         h = vp()
         rc = self.L.grains_create(C.byref(cfg), C.byref(h))
         if rc != 0:

@@ -74,6 +74,29 @@ extern "C" {
 /* Nesterov-accelerated support directions in GJK (physics semantics only). */
 #define GRAINS_NESTEROV 0x2u
 
+/* ---- time integration (grains_config_t.integrator) ------------------------ */
+/* FirstOrderExplicit::Move (FirstOrderExplicit.cpp:46-60): x += dt v,
+ * v += dt a, rotation by w over dt, w += dt alpha. */
+#define GRAINS_FIRST_ORDER_EXPLICIT 0
+/* The three further names TimeIntegratorBuilderFactory.cpp:45-49 lists (the
+ * reference implements none of them).  Same Move interface; a = F/m and
+ * alpha from computeMomentum at the step's start:
+ *  SECOND_ORDER_EXPLICIT  x += dt v + dt^2/2 a, rotation by w + dt/2 alpha,
+ *                         v += dt a, w += dt alpha: exact under constant
+ *                         accelerations, first order in general;
+ *  SECOND_ORDER_LEAPFROG  velocities live at half steps: v += dt a (dt/2 on
+ *                         the first step after grains_set_state), then
+ *                         x += dt v (the new v), rotation by the new w;
+ *                         second order for forces of the positions only;
+ *  SECOND_ORDER_ADAMS_BASHFORTH  two-step: x += dt (3/2 v - 1/2 v_prev),
+ *                         v += dt (3/2 a - 1/2 a_prev), rotation by
+ *                         3/2 w - 1/2 w_prev; the first step after
+ *                         grains_set_state (no history) is first order;
+ *                         second order. */
+#define GRAINS_SECOND_ORDER_EXPLICIT 1
+#define GRAINS_SECOND_ORDER_LEAPFROG 2
+#define GRAINS_SECOND_ORDER_ADAMS_BASHFORTH 3
+
 /* ---- pair lists ----------------------------------------------------------- */
 #define GRAINS_PAIRS_CANDIDATE 0 /* passed the bounding-sphere prefilter     */
 #define GRAINS_PAIRS_CONTACT 1   /* overlap distance < 0                     */
@@ -137,7 +160,8 @@ typedef struct grains_config_t {
                                LDS (pays off when cells hold many particles);
                                0 = 2 when particles outnumber cells 8 to 1,
                                else 1.  Results do not depend on it.        */
-  int32_t reserved[1];
+  int32_t integrator;       /* GRAINS_FIRST_ORDER_EXPLICIT (0, the reference's
+                               only scheme) or a second-order scheme below  */
 } grains_config_t;
 
 /* Counters of the last grains_step call (summed over its steps). */

@@ -1250,6 +1250,8 @@ struct go_world {
   tr3* tr;
   v3 *vel, *omega, *force, *torque;
   v3 *last_force, *last_torque;
+  v3* hist; /* 4 per particle: ADAMS_BASHFORTH, the previous v, w, a, alpha (NaN: none);
+              LEAPFROG, hist[0].x NaN until the first (half) kick */
   int32_t* result;
   /* broad phase of the last step */
   uint32_t* keys;
@@ -1283,7 +1285,7 @@ go_world* go_create(const grains_config_t* cfg) {
 static void free_particles(go_world* w) {
   free(w->shape); free(w->sim); free(w->tr); free(w->vel); free(w->omega);
   free(w->force); free(w->torque); free(w->last_force); free(w->last_torque);
-  free(w->result); free(w->keys); free(w->sorted);
+  free(w->result); free(w->keys); free(w->sorted); free(w->hist); w->hist = NULL;
   if (w->cand)
     for (uint64_t i = 0; i < w->n; ++i) { free(w->cand[i]); free(w->cont[i]); }
   free(w->cand); free(w->ncand); free(w->capcand);
@@ -1470,6 +1472,11 @@ int go_set_obstacles(go_world* w, const uint32_t* shape, const uint32_t* sim,
   return GRAINS_OK;
 }
 
+/* no history: the next two-step update is a first-order one */
+static void hist_reset(go_world* w) {
+  for (uint64_t i = 0; i < 4 * w->n; ++i) w->hist[i] = V(NAN, NAN, NAN);
+}
+
 int go_set_particles(go_world* w, const uint32_t* shape, const uint32_t* sim, uint64_t n) {
   free_particles(w);
   w->n = n;
@@ -1496,6 +1503,8 @@ int go_set_particles(go_world* w, const uint32_t* shape, const uint32_t* sim, ui
   w->pp_deep = (uint64_t*)calloc(n, sizeof(uint64_t));
   w->pp_wall = (uint64_t*)calloc(n, sizeof(uint64_t));
   w->pp_ood = (uint64_t*)calloc(n, sizeof(uint64_t));
+  w->hist = (v3*)malloc(sizeof(v3) * 4 * n);
+  hist_reset(w);
   for (uint64_t i = 0; i < n; ++i) {
     w->shape[i] = shape[i];
     if (shape[i] >= w->nshapes) return set_err(w, GRAINS_EINVAL, "shape index out of range");
@@ -1519,6 +1528,7 @@ int go_set_state(go_world* w, const double* pos, const double* quat, const doubl
     if (vel) w->vel[i] = V(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
     if (omega) w->omega[i] = V(omega[3 * i], omega[3 * i + 1], omega[3 * i + 2]);
   }
+  hist_reset(w);
   return GRAINS_OK;
 }
 
@@ -1539,6 +1549,7 @@ int go_get_state(go_world* w, double* pos, double* quat, double* vel, double* om
 int go_set_transforms(go_world* w, const double* t, uint64_t n) {
   if (n != w->n) return set_err(w, GRAINS_EINVAL, "particle count mismatch");
   for (uint64_t i = 0; i < n; ++i) w->tr[i] = tr_from12(t + 12 * i);
+  hist_reset(w);
   return GRAINS_OK;
 }
 
@@ -1689,7 +1700,9 @@ static void contacts_of(go_world* w, uint64_t i) {
 
 /* moveParticles (ComponentManagerCPU.cpp:465-500) with
  * RigidBody::computeMomentum (RigidBody.cpp:267-307),
- * FirstOrderExplicit::Move (FirstOrderExplicit.cpp:46-60),
+ * FirstOrderExplicit::Move (FirstOrderExplicit.cpp:46-60) or one of the
+ * second-order schemes named in TimeIntegratorBuilderFactory.cpp:45-49 (the
+ * reference implements none; their updates are stated in grains_hip.h),
  * TimeIntegrator::computeQuaternionChange (TimeIntegrator.cpp:28-44),
  * Transform3::updateTransform (Transform3.cpp:323-330). */
 static void move_particle(go_world* w, uint64_t i) {
@@ -1715,17 +1728,47 @@ static void move_particle(go_world* w, uint64_t i) {
   w->last_torque[i] = w->torque[i];
   w->force[i] = V(0, 0, 0);
   w->torque[i] = V(0, 0, 0);
-  /* Move */
-  v3 transMotion = vscale(dt, w->vel[i]);
-  w->vel[i] = vadd(w->vel[i], vscale(dt, accel));
+  /* Move: translation over the step, the angular velocity of the rotation */
+  const v3 vel = w->vel[i];
+  const int scheme = w->cfg.integrator;
+  v3 transMotion, wr;
+  v3* h = w->hist + 4 * i;
+  if (scheme == GRAINS_SECOND_ORDER_EXPLICIT) {
+    transMotion = vadd(vscale(dt, vel), vscale(0.5 * dt * dt, accel));
+    wr = vadd(om, vscale(0.5 * dt, alpha));
+    w->vel[i] = vadd(vel, vscale(dt, accel));
+    w->omega[i] = vadd(om, vscale(dt, alpha));
+  } else if (scheme == GRAINS_SECOND_ORDER_LEAPFROG) {
+    const double kick = isnan(h[0].x) ? 0.5 * dt : dt; /* the first step: a half kick */
+    w->vel[i] = vadd(vel, vscale(kick, accel));
+    w->omega[i] = vadd(om, vscale(kick, alpha));
+    transMotion = vscale(dt, w->vel[i]);
+    wr = w->omega[i];
+    h[0].x = 0.0;
+  } else if (scheme == GRAINS_SECOND_ORDER_ADAMS_BASHFORTH && !isnan(h[2].x)) {
+    transMotion = vscale(dt, vsub(vscale(1.5, vel), vscale(0.5, h[0])));
+    wr = vsub(vscale(1.5, om), vscale(0.5, h[1]));
+    w->vel[i] = vadd(vel, vscale(dt, vsub(vscale(1.5, accel), vscale(0.5, h[2]))));
+    w->omega[i] = vadd(om, vscale(dt, vsub(vscale(1.5, alpha), vscale(0.5, h[3]))));
+  } else {
+    transMotion = vscale(dt, vel);
+    wr = om;
+    w->vel[i] = vadd(vel, vscale(dt, accel));
+    w->omega[i] = vadd(om, vscale(dt, alpha));
+  }
+  if (scheme == GRAINS_SECOND_ORDER_ADAMS_BASHFORTH) {
+    h[0] = vel;
+    h[1] = om;
+    h[2] = accel;
+    h[3] = alpha;
+  }
   qt dq = {0.0, 0.0, 0.0, 1.0};
-  double nOmega = vnorm(om);
+  double nOmega = vnorm(wr);
   if (nOmega > HIGHEPS) {
     double c = cos(nOmega * dt / 2.0);
     double s = sin(nOmega * dt / 2.0);
-    dq = Q(vscale(s / nOmega, om), c);
+    dq = Q(vscale(s / nOmega, wr), c);
   }
-  w->omega[i] = vadd(om, vscale(dt, alpha));
   /* updateTransform */
   w->tr[i].o = vadd(w->tr[i].o, transMotion);
   compose_left_by_rotation(&w->tr[i].R, dq);

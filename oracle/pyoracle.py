@@ -79,10 +79,12 @@ def _c(a, dtype=np.float64):
 class Oracle:
     """FP64 CPU oracle world; method names follow grains3dgpu_amd.engine.Engine."""
 
-    def __init__(self, dt=1e-6, precision=A.FP64, flags=0, gjk_rel_tol=0.0, threads=0):
+    def __init__(self, dt=1e-6, precision=A.FP64, flags=0, gjk_rel_tol=0.0, threads=0,
+                 integrator=A.FIRST_ORDER_EXPLICIT):
         self.L = lib()
         cfg = A.Config()
         cfg.dt, cfg.precision, cfg.flags, cfg.gjk_rel_tol = dt, precision, flags, gjk_rel_tol
+        cfg.integrator = integrator
         self.w = self.L.go_create(C.byref(cfg))
         self.threads = threads
         self.n = 0

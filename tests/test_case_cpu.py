@@ -237,3 +237,18 @@ def test_run_case_writes_paraview(deck):
     # 3 spheres, 2 boxes, 1 superquadric
     assert int(piece.get("NumberOfPoints")) == 3 * 483 + 2 * 8 + 243
     assert int(piece.get("NumberOfCells")) == 3 * 512 + 2 * 1 + 256
+
+
+@pytest.mark.parametrize("name,code", [("FirstOrderExplicit", A.FIRST_ORDER_EXPLICIT),
+                                       ("SecondOrderLeapFrog", A.SECOND_ORDER_LEAPFROG),
+                                       ("SecondOrderAdamsBashforth", A.SECOND_ORDER_ADAMS_BASHFORTH),
+                                       ("RungeKutta4", None)])
+def test_time_integration_types(tmp_path, name, code):
+    """TimeIntegration Type: the four names of TimeIntegratorBuilderFactory.cpp:39-49."""
+    deck = DECK.replace('<TimeIntegration Type="FirstOrderExplicit"/>', f'<TimeIntegration Type="{name}"/>')
+    (tmp_path / "d.xml").write_text(deck)
+    if code is None:
+        with pytest.raises(C.CaseError, match="RungeKutta4"):
+            C.read_case(str(tmp_path / "d.xml"))
+    else:
+        assert C.read_case(str(tmp_path / "d.xml")).integrator == code
