@@ -80,6 +80,10 @@ def log(*a):
 # 1e-6 (FP32)
 GJK_TOL = {A.FP64: 1e-10, A.FP32_MIXED: 1e-6}
 
+# FP32-mixed runs start each pair's GJK search from the pair's direction of
+# the previous step (GRAINS_WARM_START); the cold start is timed beside it
+FP32_FLAGS = A.WARM_START
+
 
 MAX_STATS = ("max_candidates", "gjk_max_iterations", "sort_passes")
 
@@ -304,9 +308,12 @@ def bench_c1(args):
     log(f"FP32-mixed: {W} warm-up + profiled + {K} timed steps")
     # a few residents per cell: the direct candidate search (what the
     # automatic choice takes here)
-    r32 = run_engine(sc, A.FP32_MIXED, W, K, candidate_staging=1)
+    r32 = run_engine(sc, A.FP32_MIXED, W, K, flags=FP32_FLAGS, candidate_staging=1)
     ps32 = sc.n * K / r32["wall_s"]
     log(f"  {r32['wall_s'] * 1e3 / K:.3f} ms/step, {ps32:.3e} particle-steps/s")
+    log(f"FP32-mixed, cold GJK starts: {W} warm-up + {K} timed steps")
+    r32c = run_engine(sc, A.FP32_MIXED, W, K, profile=False, candidate_staging=1)
+    log(f"  {r32c['wall_s'] * 1e3 / K:.3f} ms/step")
     r64 = None
     K64 = args.fp64_steps or K
     if not args.no_fp64:
@@ -317,7 +324,7 @@ def bench_c1(args):
     # error of FP32-mixed against FP64 after the same W + K64 steps
     err = None
     if r64 is not None:
-        p32 = run_engine(sc, A.FP32_MIXED, W, K64, profile=False)
+        p32 = run_engine(sc, A.FP32_MIXED, W, K64, profile=False, flags=FP32_FLAGS)
         p64 = run_engine(sc, A.FP64, W, K64, profile=False)
         err = dict(steps=W + K64, **pos_error(p32, p64, sc.pos))
 
@@ -344,7 +351,7 @@ def bench_c1(args):
         sa.vel = rng.normal(0.0, 0.1, (na, 3))
         sa.omega = rng.normal(0.0, 100.0, (na, 3))
         log(f"agitated bed, {na} particles: FP32-mixed and FP64, {W + K} steps each")
-        a32 = run_engine(sa, A.FP32_MIXED, W, K, profile=False)
+        a32 = run_engine(sa, A.FP32_MIXED, W, K, profile=False, flags=FP32_FLAGS)
         a64 = run_engine(sa, A.FP64, W, K, profile=False)
         # the same FP64 path stopped at the FP32 path's GJK gap: the part of
         # the divergence that the gap alone causes in this chaotic flow
@@ -376,6 +383,10 @@ def bench_c1(args):
             "particle_steps_per_s": sc.n * K64 / r64["wall_s"], "ms_per_step": r64["wall_s"] * 1e3 / K64,
             "gjk_pair_tests_per_s": r64["stats"]["gjk_calls"] / r64["wall_s"],
             "speedup_fp32_mixed_over_fp64": ps32 / (sc.n * K64 / r64["wall_s"])},
+        fp32_mixed_cold_gjk={"particle_steps_per_s": sc.n * K / r32c["wall_s"],
+                             "ms_per_step": r32c["wall_s"] * 1e3 / K,
+                             "gjk_iterations": r32c["stats"]["gjk_iterations"],
+                             "gjk_iterations_warm": st["gjk_iterations"]},
         fp32_mixed_error_vs_fp64=err,
         fp32_mixed_error_vs_fp64_agitated=agit,
         gjk_pair_tests_per_s=st["gjk_calls"] / r32["wall_s"],
@@ -407,7 +418,7 @@ def bench_c2(args):
     log(f"FP32-mixed: {W} warm-up + profiled + {K} timed steps")
     # ~145 residents per cell: the LDS-staged candidate search (what the
     # automatic choice takes here)
-    r32 = run_engine(sc, A.FP32_MIXED, W, K, max_neighbors=96, candidate_staging=2)
+    r32 = run_engine(sc, A.FP32_MIXED, W, K, flags=FP32_FLAGS, max_neighbors=96, candidate_staging=2)
     ps32 = sc.n * K / r32["wall_s"]
     log(f"  {r32['wall_s'] * 1e3 / K:.3f} ms/step, {ps32:.3e} particle-steps/s")
     r64 = None
@@ -456,17 +467,18 @@ def bench_c3(args):
     sc = S.config3(log=None)
     log(f"  done in {time.perf_counter() - t0:.1f} s: {sc.n} particles, rounds = {sc.notes['rounds']}")
     t0 = time.perf_counter()
-    tuned = autotune(sc, precision=A.FP32_MIXED, gjk_rel_tol=GJK_TOL[A.FP32_MIXED], steps=10, log=log)
+    tuned = autotune(sc, precision=A.FP32_MIXED, flags=FP32_FLAGS, gjk_rel_tol=GJK_TOL[A.FP32_MIXED], steps=10,
+                     log=log)
     tune_s = time.perf_counter() - t0
     log(f"  autotuner: {tuned['cell_size_factor']}, {tuned['block_size']}, {tuned['narrow_blocks_per_cu']}, "
         f"{tuned['narrow_passes']} in {tune_s:.1f} s")
     launch = dict(block_size=tuned["block_size"], narrow_blocks_per_cu=tuned["narrow_blocks_per_cu"],
                   narrow_passes=tuned["narrow_passes"], candidate_staging=tuned["candidate_staging"])
-    r32 = run_engine(sc, A.FP32_MIXED, W, K, **launch)
+    r32 = run_engine(sc, A.FP32_MIXED, W, K, flags=FP32_FLAGS, **launch)
     ps32 = sc.n * K / r32["wall_s"]
     log(f"  {r32['wall_s'] * 1e3 / K:.3f} ms/step, {ps32:.3e} particle-steps/s")
     # the untuned default launch beside it
-    d32 = run_engine(sc, A.FP32_MIXED, W, min(K, 200), profile=False)
+    d32 = run_engine(sc, A.FP32_MIXED, W, min(K, 200), profile=False, flags=FP32_FLAGS)
     cpu = None
     if not args.no_cpu:
         k = int(np.argmin(np.abs(np.array(sc.notes["sizes"]) - 5000)))

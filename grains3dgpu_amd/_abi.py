@@ -14,6 +14,7 @@ SPHERE, BOX, CYLINDER, CONE, SUPERQUADRIC, POLYHEDRON, RECTANGLE = range(7)
 FP64, FP32_MIXED = 0, 1
 REF_QUIRKS = 0x1
 NESTEROV = 0x2
+WARM_START = 0x4
 
 # grains_config_t.integrator
 FIRST_ORDER_EXPLICIT, SECOND_ORDER_EXPLICIT, SECOND_ORDER_LEAPFROG, SECOND_ORDER_ADAMS_BASHFORTH = range(4)

@@ -32,11 +32,14 @@ void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
 }
 
 // ---- cell hash --------------------------------------------------------------------
+// ws_par (warm starts on): the step's warm-start buffer index, flipped here,
+// the first kernel of a step (no other kernel runs beside it on the stream).
 __global__ void k_hash(uint64_t n, const double* __restrict__ pos, const uint32_t* __restrict__ sim,
                        const SimGrid* __restrict__ grids, uint32_t* __restrict__ key,
-                       unsigned long long* __restrict__ counters) {
+                       unsigned long long* __restrict__ counters, uint32_t* __restrict__ ws_par) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t ood = 0;
+  if (ws_par && i == 0) ws_par[0] = 1u - ws_par[0];
   if (i < n) {
     const SimGrid& g = grids[sim[i]];
     const double p[3] = {pos[i], pos[n + i], pos[2 * n + i]};
@@ -61,7 +64,7 @@ __global__ void k_hash(uint64_t n, const double* __restrict__ pos, const uint32_
 
 void launch_hash(const DevScene& d, hipStream_t s) {
   hipLaunchKernelGGL(k_hash, dim3((unsigned)((d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d.n,
-                     d.pos, d.sim, d.grids, d.key, d.counters);
+                     d.pos, d.sim, d.grids, d.key, d.counters, d.warm ? d.ws_par : nullptr);
 }
 
 // ---- block-wide exclusive scan of one value per thread ---------------------------

@@ -555,6 +555,16 @@ static int hist_reset(grains_ctx* ctx) {
   return GRAINS_OK;
 }
 
+// Warm-start records forgotten: the next step searches cold (a new state is
+// not the continuation of the recorded directions).
+static int warm_reset(grains_ctx* ctx) {
+  if (!ctx->d.ws_n) return GRAINS_OK;
+  launch_fill_u32(ctx->d.ws_n, 2 * ctx->n, 0u, ctx->stream);
+  GR_HIP(hipGetLastError());
+  GR_HIP(hipStreamSynchronize(ctx->stream));
+  return GRAINS_OK;
+}
+
 static int alloc_state(grains_ctx* ctx) {
   free_list(ctx->state_allocs);
   const uint64_t n = ctx->n;
@@ -787,6 +797,22 @@ static int finalize(grains_ctx* ctx) {
       (rc = dalloc(ctx, &d.defer_flat, (ecap + 1) * 4, L)) ||
       (rc = dalloc(ctx, &d.counters, (size_t)kNumCounters * kCntSlots * 8, L)))
     return rc;
+  // warm-start records: two buffers of ecap = K x N, zeroed (no direction)
+  d.warm = ((ctx->cfg.flags & GRAINS_WARM_START) && ctx->cfg.precision == GRAINS_FP32_MIXED &&
+            !(ctx->cfg.flags & GRAINS_REF_QUIRKS))
+               ? 1
+               : 0;
+  d.ws = nullptr;
+  d.ws_n = nullptr;
+  d.ws_par = nullptr;
+  if (d.warm) {
+    if ((rc = dalloc(ctx, &d.ws, 2 * ecap * sizeof(float4), L)) || (rc = dalloc(ctx, &d.ws_n, 2 * n * 4, L)) ||
+        (rc = dalloc(ctx, &d.ws_par, 16, L)))
+      return rc;
+    launch_fill_u32((uint32_t*)d.ws, 2 * ecap * 4, 0u, ctx->stream);
+    launch_fill_u32(d.ws_n, 2 * n, 0u, ctx->stream);
+    launch_fill_u32(d.ws_par, 4, 0u, ctx->stream);
+  }
   GR_HIP(hipMemcpy(p64, np64.data(), S * sizeof(ShapeNP<double>), hipMemcpyHostToDevice));
   GR_HIP(hipMemcpy(p32, np32.data(), S * sizeof(ShapeNP<float>), hipMemcpyHostToDevice));
   GR_HIP(hipMemcpy(pdyn, dyn.data(), S * sizeof(ShapeDyn), hipMemcpyHostToDevice));
@@ -855,7 +881,7 @@ extern "C" int grains_set_state(grains_ctx_t* ctx, const double* pos, const doub
   };
   int rc;
   if ((rc = put(pos, ctx->d.pos, 3)) || (rc = put(quat, ctx->d.quat, 4)) || (rc = put(vel, ctx->d.vel, 3)) ||
-      (rc = put(omega, ctx->d.omega, 3)))
+      (rc = put(omega, ctx->d.omega, 3)) || (rc = warm_reset(ctx)))
     return rc;
   return hist_reset(ctx);
 }
@@ -1395,7 +1421,7 @@ extern "C" int grains_restore(grains_ctx_t* ctx, const void* bytes, size_t size)
   GR_HIP(hipMemcpy(ctx->d.contact_count, p, n * 4, hipMemcpyHostToDevice));
   p += n * 4;
   if (nh) GR_HIP(hipMemcpy(ctx->d.ab_hist, p, nh, hipMemcpyHostToDevice));
-  return GRAINS_OK;
+  return warm_reset(ctx);
 }
 
 // ---- differentiation: the reverse sweep ---------------------------------------------
@@ -1416,6 +1442,8 @@ extern "C" int grains_adjoint_record(grains_ctx_t* ctx, uint32_t checkpoint_ever
   if (ctx->d.gen_obs) return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers Rectangle obstacles");
   if (ctx->cfg.integrator != GRAINS_FIRST_ORDER_EXPLICIT)
     return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint covers FirstOrderExplicit");
+  if (ctx->d.warm)
+    return fail(ctx, GRAINS_EUNSUPPORTED, "the adjoint replays steps from checkpoints: no GRAINS_WARM_START");
   // the sweep's buffers are sized by checkpoint_every: drop them
   free_list(ctx->adj_cp);
   free_list(ctx->adj_allocs);

@@ -426,6 +426,9 @@ struct GjkOut {
                   // the search ended on rounding or the cap, not the gap test
   bool separated; // signed volumes: the lower bound passed `sep` (no contact)
   bool deferred;  // screening search: not separated by the first support pair
+  V3<T> vdir;     // signed volumes: the search's last direction in A's frame
+                  // (the closest-point direction of a converged search)
+  bool has_v;     // vdir is a direction (not inside, not degenerate)
 };
 
 // GJK iteration budget of a complete search
@@ -694,9 +697,15 @@ constexpr int kNesterovIters = 8;
 // from the start).  Everything up to that test is this same code, so the
 // screen separates exactly the pairs the full search separates at its first
 // support pair.
+// warm (optional): a first search direction in A's frame, e.g. the pair's
+// closest-point direction of the previous step.  Its support pair seeds the
+// simplex (v = w, |v| and the lower bound from it); a search whose gap test
+// already holds there ends after that one support pair.  The direction only
+// steers the search: the distance, witnesses and decisions meet the same
+// gap test as from the centre difference.
 template <class T, bool SCREEN = false>
 GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, const Tr<T>& b2w,
-                      const GjkArgs& args, T rsum, T sep) {
+                      const GjkArgs& args, T rsum, T sep, const V3<T>* warm = nullptr) {
   const bool nesterov = (args.flags & GRAINS_NESTEROV) != 0;
   const T rel = args.rel > 0 ? T(args.rel) : Eps<T>::LOW;
   Tr<T> b2a = relative(a2w, b2w);
@@ -715,10 +724,40 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   int n = 0;
   V3<T> v = -b2a.o, w = zero, pl = zero, ql = zero;
   T dist = norm(v), mu = T(0);
-  V3<T> dir = v, wprev = v;
+  V3<T> dir = v, wprev = v, last_d = v;
   bool accel = nesterov, inside = false, ran = false, stalled = false, separated = false, done = false;
+  bool screened = false;
   int num_iter = 0;
-  while (dist > Eps<T>::HIGH && num_iter < kGjkMaxIter) {
+  if (warm) {
+    const V3<T> d0 = *warm;
+    last_d = d0;
+    pl = sc * support(A, -d0);
+    ql = sc * support(B, vm(d0, b2a.R));
+    w = pl - apply(b2a, ql);
+    ran = true;
+    accel = false;
+    const T lb = dot(d0, w) / norm(d0);
+    if (mu < lb) mu = lb;
+    if (mu > sep_s) {
+      separated = true;
+      done = true;
+    } else if (SCREEN) {
+      screened = true;
+    } else {
+      S.y[0] = w;
+      S.p[0] = pl;
+      S.q[0] = ql;
+      S.lam[0] = T(1);
+      n = 1;
+      num_iter = 1;
+      v = w;
+      dist = norm(w);
+      wprev = w;
+      dir = w;
+      if (dist - mu <= dist * rel) done = true;
+    }
+  }
+  while (!done && !screened && dist > Eps<T>::HIGH && num_iter < kGjkMaxIter) {
     V3<T> d = v;
     if (accel && num_iter >= kNesterovIters) accel = false;
     if (accel) {
@@ -733,6 +772,7 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
     ql = sc * support(B, vm(d, b2a.R));
     w = pl - apply(b2a, ql);
     ran = true;
+    last_d = d;
     const T lb = accel ? dot(d, w) / norm(d) : dot(v, w) / dist;
     if (mu < lb) mu = lb;
     if (mu > sep_s) {
@@ -847,6 +887,10 @@ GR_D GjkOut<T> gjk_sv(const SupCtx<T>& A, const SupCtx<T>& B, const Tr<T>& a2w, 
   out.backup = stalled;
   out.separated = separated;
   out.deferred = deferred;
+  // the direction to start from next time: the converged closest-point
+  // direction, else the last one searched
+  out.vdir = (done && !separated && !inside && n > 0 && dist > Eps<T>::HIGH) ? v : last_d;
+  out.has_v = !inside && norm(out.vdir) > T(0);
   return out;
 }
 
@@ -899,10 +943,14 @@ struct NpStats {
 // SCREEN (physics semantics): decide only the pairs without contact that the
 // bounding sphere, the OBB or GJK's first support pair settle; every other
 // pair returns st.deferred for the full pass.
+// warm_w (optional, physics semantics): a first GJK direction in the world
+// frame; dir_w (optional): receives the search's direction in the world frame
+// when GJK ran and produced one (left alone otherwise).
 template <class T, bool SV, bool SCREEN = false>
 GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<T>& a2w,
                              const Tr<T>& b2w, const GjkArgs& args, const T* verts_full,
-                             const T* verts_core, NpStats& st) {
+                             const T* verts_core, NpStats& st, const V3<T>* warm_w = nullptr,
+                             V3<T>* dir_w = nullptr) {
   constexpr bool quirks = !SV;
   if ((A.kind == GRAINS_SPHERE && B.kind == GRAINS_SPHERE) || B.kind == GRAINS_RECTANGLE) {
     if constexpr (SCREEN) {
@@ -919,10 +967,17 @@ GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<
       SupCtx<T> cb = sup_ctx(B, !quirks, verts_full, verts_core);
       const T ctSum = A.crust + B.crust;
       GjkOut<T> g;
-      if constexpr (SV)
-        g = gjk_sv<T, SCREEN>(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum);
-      else
+      if constexpr (SV) {
+        if (warm_w) {
+          const V3<T> wa = vm(*warm_w, a2w.R);  // into A's frame
+          g = gjk_sv<T, SCREEN>(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum, &wa);
+        } else {
+          g = gjk_sv<T, SCREEN>(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum);
+        }
+        if (dir_w && g.has_v) *dir_w = mv(a2w.R, g.vdir);
+      } else {
         g = gjk_jh(ca, cb, a2w, b2w, args);
+      }
       if (g.deferred) {
         st.deferred = 1;
         return no_contact<T>();

@@ -47,9 +47,11 @@ __device__ __forceinline__ void pair_frames(const SimGrid& g, V3<double> xi, Qt<
 // si's side (ComponentManagerCPU.cpp:384-419).  Poses come from the sorted
 // records; velocities, needed only for contacts, from the planes through the
 // particle ids.  Returns the pair flag.
-template <class T, class R, bool SV, bool SCREEN = false>
+// e: the entry (its slot in si's candidate list keys the warm-start record);
+// ~0u for callers without one (the FP64 re-run).
+template <class T, class R, bool SV, bool SCREEN = false, bool WARM = false>
 __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, uint32_t sj, PairRes<R>& out,
-                                               NpStats& st) {
+                                               NpStats& st, uint32_t e = ~0u) {
   const uint64_t n = d.n;
   const PoseRec pi = d.pk[si];
   const PoseRec pj = d.pk[sj];
@@ -62,7 +64,40 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
   pair_frames<T>(g, xi, Qt<double>{pi.qx, pi.qy, pi.qz, pi.qw}, V3<double>{pj.x, pj.y, pj.z},
                  Qt<double>{pj.qx, pj.qy, pj.qz, pj.qw}, a2w, b2w);
   const GjkArgs args = {d.flags, d.gjk_rel};
-  CInfo<T> ci = closest_points<T, SV, SCREEN>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
+  CInfo<T> ci;
+  if constexpr (sizeof(T) == 4 && SV && WARM) {
+    if (e != ~0u) {
+      // the previous step's direction of this (i, j): slot k of i's list
+      // first (lists keep their order while neighbours stay), then a scan
+      const uint64_t KN = (uint64_t)d.K * n;
+      const uint32_t cur = *d.ws_par;
+      const uint32_t wi = d.sidx[si], wj = d.sidx[sj], wk = e - d.entry_off[si];
+      const float4* prev = d.ws + (uint64_t)(1u - cur) * KN;
+      const uint32_t np = d.ws_n[(uint64_t)(1u - cur) * n + wi];
+      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+      bool hit = false;
+      if (wk < np) {
+        r = prev[(uint64_t)wk * n + wi];
+        hit = __float_as_uint(r.w) == wj;
+      }
+      for (uint32_t k = 0; !hit && k < np; ++k) {
+        r = prev[(uint64_t)k * n + wi];
+        hit = __float_as_uint(r.w) == wj;
+      }
+      const V3<T> wv = {r.x, r.y, r.z};
+      const T nn = dot(wv, wv);
+      const bool use = hit && nn > T(0) && nn < T(INFINITY);
+      V3<T> dir = {T(0), T(0), T(0)};
+      ci = closest_points<T, SV, SCREEN>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st,
+                                         use ? &wv : nullptr, &dir);
+      if (!st.deferred)
+        d.ws[(uint64_t)cur * KN + (uint64_t)wk * n + wi] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(wj));
+    } else {
+      ci = closest_points<T, SV, SCREEN>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
+    }
+  } else {
+    ci = closest_points<T, SV, SCREEN>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
+  }
   uint32_t flag = (st.gjk ? kFlagGjk : 0u) | (st.deep ? kFlagDeep : 0u);
   if constexpr (SCREEN) return flag;  // decided pairs have no contact
   if (ci.overlap < T(0)) {
@@ -113,7 +148,8 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
 // atomic: a single queue counter would serialise one atomic per wave); the
 // segments are copied into one list at the offsets of an exclusive scan of
 // their counts (launch_defer_flatten), which PASS 1 reads in order.
-template <class T, bool SV, int PASS>
+// WARM: FP32 searches start from the pair's previous direction (d.warm).
+template <class T, bool SV, int PASS, bool WARM = false>
 __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
   constexpr bool screen = SV && PASS == 2;
   __shared__ uint32_t seg_n;
@@ -131,7 +167,7 @@ __global__ void __launch_bounds__(kBlock) k_narrow(DevScene d) {
     if constexpr (PASS == 1) e = d.defer_flat[k];
     NpStats st = {0, 0, 0, 0, 0};
     PairRes<T> r;
-    uint32_t flag = pair_force<T, T, SV, screen>(d, d.pair_i[e], d.pair_j[e], r, st);
+    uint32_t flag = pair_force<T, T, SV, screen, WARM>(d, d.pair_i[e], d.pair_j[e], r, st, e);
     if constexpr (screen) block_append(seg, &seg_n, st.deferred != 0, e);
     if (!st.deferred) {
       if (flag & kFlagContact) res[e] = r;
@@ -214,6 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
     const PairRes<T>* res = (const PairRes<T>*)d.pair_res;
     const uint32_t s = d.rank[i];
     const uint32_t e0 = d.entry_off[s], e1 = d.entry_off[s + 1];
+    if (d.warm) d.ws_n[(uint64_t)(*d.ws_par) * n + i] = e1 - e0;  // the warm-start records of i this step
     V3<double> F = {0.0, 0.0, 0.0}, M = {0.0, 0.0, 0.0};
     int32_t ncon = 0;
     for (uint32_t e = e0; e < e1; ++e) {

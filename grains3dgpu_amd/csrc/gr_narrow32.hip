@@ -7,16 +7,25 @@ namespace gr {
 void launch_narrow32(const DevScene& d, int pass, hipStream_t s) {
   const bool sv = !(d.flags & GRAINS_REF_QUIRKS);
   const bool two = sv && d.split > 0;
+  const bool warm = sv && d.warm;
+  const dim3 grid(d.narrow_blocks), block(d.block);
   if (pass == 0) {
-    if (two)
-      hipLaunchKernelGGL((k_narrow<float, true, 2>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+    if (two && warm)
+      hipLaunchKernelGGL((k_narrow<float, true, 2, true>), grid, block, 0, s, d);
+    else if (two)
+      hipLaunchKernelGGL((k_narrow<float, true, 2>), grid, block, 0, s, d);
+    else if (warm)
+      hipLaunchKernelGGL((k_narrow<float, true, 0, true>), grid, block, 0, s, d);
     else if (sv)
-      hipLaunchKernelGGL((k_narrow<float, true, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+      hipLaunchKernelGGL((k_narrow<float, true, 0>), grid, block, 0, s, d);
     else
-      hipLaunchKernelGGL((k_narrow<float, false, 0>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+      hipLaunchKernelGGL((k_narrow<float, false, 0>), grid, block, 0, s, d);
   } else if (two) {
     launch_defer_flatten(d, s);
-    hipLaunchKernelGGL((k_narrow<float, true, 1>), dim3(d.narrow_blocks), dim3(d.block), 0, s, d);
+    if (warm)
+      hipLaunchKernelGGL((k_narrow<float, true, 1, true>), grid, block, 0, s, d);
+    else
+      hipLaunchKernelGGL((k_narrow<float, true, 1>), grid, block, 0, s, d);
   }
 }
 void launch_integrate32(const DevScene& d, hipStream_t s) {

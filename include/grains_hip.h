@@ -73,6 +73,15 @@ extern "C" {
 #define GRAINS_REF_QUIRKS 0x1u
 /* Nesterov-accelerated support directions in GJK (physics semantics only). */
 #define GRAINS_NESTEROV 0x2u
+/* FP32-mixed, physics semantics: every GJK search of a particle pair starts
+ * from that pair's search direction of the previous step (kept on the device
+ * per (particle, candidate slot): 32 B x max_neighbors per particle).  A
+ * resting or slowly moving pair then ends after one support pair.  Results
+ * meet the same gap test as cold searches but depend on the previous step:
+ * grains_set_state, grains_set_transforms and grains_restore start cold,
+ * so a restored run differs from an uninterrupted one in the last bits.
+ * Ignored under GRAINS_FP64 and GRAINS_REF_QUIRKS; the adjoint rejects it. */
+#define GRAINS_WARM_START 0x4u
 
 /* ---- time integration (grains_config_t.integrator) ------------------------ */
 /* FirstOrderExplicit::Move (FirstOrderExplicit.cpp:46-60): x += dt v,
