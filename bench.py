@@ -61,6 +61,10 @@ B_NP_PARTICLE = 64
 B_NP_SHAPE = 96
 B_NP_ENTRY = 12
 B_NP_CONTACT = 24
+# warm-started GJK: the pair's previous direction record (16 B) read and the
+# new one written, per candidate entry, plus each particle's record count
+B_NP_WARM_ENTRY = 32
+B_NP_WARM_PARTICLE = 8
 # Compulsory HBM bytes of one candidate-search launch: each particle's FP64
 # record (32) and FP32 record (16), key, sim and candidate count (12) once;
 # each candidate written (4)
@@ -207,21 +211,25 @@ def measured_traffic(traffic_file):
     return out
 
 
-def narrow_roofline(sc, r, K, traffic=None):
-    """Roofline of the screening narrow pass (k_narrow<float,true,2>, one
-    launch per step): compulsory bytes per launch over its HIP-event time."""
+def narrow_roofline(sc, r, K, traffic=None, warm=True):
+    """Roofline of the screening narrow pass (k_narrow<float,true,2,warm>,
+    one launch per step): compulsory bytes per launch over its HIP-event
+    time."""
     st = r["stats"]
     nar_ms = r["phases"]["narrow"]
     E, C = st["candidate_pairs"] / K, st["contacts"] / K
     nbytes = sc.n * B_NP_PARTICLE + shape_bytes(sc) + E * B_NP_ENTRY + C * B_NP_CONTACT
+    if warm:
+        nbytes += E * B_NP_WARM_ENTRY + sc.n * B_NP_WARM_PARTICLE
     achieved_gbs = nbytes / (nar_ms * 1e-3) / 1e9
-    kernel = "k_narrow<float,true,2>"
+    kernel = "k_narrow<float,true,2,true>" if warm else "k_narrow<float,true,2,false>"
     return dict(bound="hbm", kernel=kernel, achieved=achieved_gbs, peak=PEAK_HBM_GBS,
                 unit="GB/s", frac=achieved_gbs / PEAK_HBM_GBS, traffic=(traffic or {}).get(kernel),
                 bytes_per_launch=nbytes, avg_launch_ms=nar_ms,
                 note="screening narrow pass (every candidate entry); compulsory bytes: 64 B pose+ids per "
                      "particle, 96 B per shape record (+12 B per polyhedron vertex), 12 B per candidate "
-                     "entry, 24 B per contact; the launch is gather-latency bound")
+                     "entry, 24 B per contact" + (", 32 B per entry and 8 B per particle of warm-start "
+                                                   "records" if warm else "") + "; the launch is gather-latency bound")
 
 
 def candidates_roofline(sc, r, K, staged, traffic=None):

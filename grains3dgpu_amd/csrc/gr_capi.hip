@@ -558,8 +558,8 @@ static int hist_reset(grains_ctx* ctx) {
 // Warm-start records forgotten: the next step searches cold (a new state is
 // not the continuation of the recorded directions).
 static int warm_reset(grains_ctx* ctx) {
-  if (!ctx->d.ws_n) return GRAINS_OK;
-  launch_fill_u32(ctx->d.ws_n, 2 * ctx->n, 0u, ctx->stream);
+  if (!ctx->d.ws_nb) return GRAINS_OK;
+  launch_fill_u32((uint32_t*)ctx->d.ws_nb, 4 * ctx->n, 0u, ctx->stream);
   GR_HIP(hipGetLastError());
   GR_HIP(hipStreamSynchronize(ctx->stream));
   return GRAINS_OK;
@@ -803,14 +803,14 @@ static int finalize(grains_ctx* ctx) {
                ? 1
                : 0;
   d.ws = nullptr;
-  d.ws_n = nullptr;
+  d.ws_nb = nullptr;
   d.ws_par = nullptr;
   if (d.warm) {
-    if ((rc = dalloc(ctx, &d.ws, 2 * ecap * sizeof(float4), L)) || (rc = dalloc(ctx, &d.ws_n, 2 * n * 4, L)) ||
+    if ((rc = dalloc(ctx, &d.ws, 2 * ecap * sizeof(float4), L)) || (rc = dalloc(ctx, &d.ws_nb, 2 * n * sizeof(uint2), L)) ||
         (rc = dalloc(ctx, &d.ws_par, 16, L)))
       return rc;
     launch_fill_u32((uint32_t*)d.ws, 2 * ecap * 4, 0u, ctx->stream);
-    launch_fill_u32(d.ws_n, 2 * n, 0u, ctx->stream);
+    launch_fill_u32((uint32_t*)d.ws_nb, 4 * n, 0u, ctx->stream);
     launch_fill_u32(d.ws_par, 4, 0u, ctx->stream);
   }
   GR_HIP(hipMemcpy(p64, np64.data(), S * sizeof(ShapeNP<double>), hipMemcpyHostToDevice));

@@ -943,13 +943,19 @@ struct NpStats {
 // SCREEN (physics semantics): decide only the pairs without contact that the
 // bounding sphere, the OBB or GJK's first support pair settle; every other
 // pair returns st.deferred for the full pass.
-// warm_w (optional, physics semantics): a first GJK direction in the world
-// frame; dir_w (optional): receives the search's direction in the world frame
-// when GJK ran and produced one (left alone otherwise).
-template <class T, bool SV, bool SCREEN = false>
+// warm (physics semantics): asked for a first GJK direction in the world
+// frame only when the pair reaches GJK (fetch returns false: none); dir_w
+// (optional): receives the search's direction in the world frame when GJK
+// ran and produced one (left alone otherwise).
+struct NoWarm {
+  template <class V>
+  GR_D bool fetch(V&) const { return false; }
+};
+
+template <class T, bool SV, bool SCREEN = false, class W = NoWarm>
 GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<T>& a2w,
                              const Tr<T>& b2w, const GjkArgs& args, const T* verts_full,
-                             const T* verts_core, NpStats& st, const V3<T>* warm_w = nullptr,
+                             const T* verts_core, NpStats& st, const W& warm = W(),
                              V3<T>* dir_w = nullptr) {
   constexpr bool quirks = !SV;
   if ((A.kind == GRAINS_SPHERE && B.kind == GRAINS_SPHERE) || B.kind == GRAINS_RECTANGLE) {
@@ -968,8 +974,9 @@ GR_D CInfo<T> closest_points(const ShapeNP<T>& A, const ShapeNP<T>& B, const Tr<
       const T ctSum = A.crust + B.crust;
       GjkOut<T> g;
       if constexpr (SV) {
-        if (warm_w) {
-          const V3<T> wa = vm(*warm_w, a2w.R);  // into A's frame
+        V3<T> warm_w;
+        if (warm.fetch(warm_w)) {
+          const V3<T> wa = vm(warm_w, a2w.R);  // into A's frame
           g = gjk_sv<T, SCREEN>(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum, &wa);
         } else {
           g = gjk_sv<T, SCREEN>(ca, cb, a2w, b2w, args, A.rc + B.rc, ctSum);

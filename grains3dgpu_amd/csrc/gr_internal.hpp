@@ -149,12 +149,13 @@ struct DevScene {
   double* ab_hist;
   // GRAINS_WARM_START (FP32-mixed, physics semantics): each entry's GJK
   // direction of the step, in the world frame, in two buffers of K x N
-  // records {x, y, z, neighbour id as bits} by (slot k, particle id), and the
-  // entries per particle of each; ws_par = the buffer this step writes
-  // (flipped at the start of every step), the other holds the previous step
+  // records {x, y, z, neighbour id as bits} in entry order, and per particle
+  // id the first entry and the entry count of each step (ws_nb); ws_par =
+  // the buffer this step writes (flipped at the start of every step), the
+  // other holds the previous step
   int32_t warm;
   float4* ws;
-  uint32_t* ws_n;
+  uint2* ws_nb;
   uint32_t* ws_par;
   // per particle outputs
   int32_t* contact_count;

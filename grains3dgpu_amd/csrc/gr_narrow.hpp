@@ -43,6 +43,44 @@ __device__ __forceinline__ void pair_frames(const SimGrid& g, V3<double> xi, Qt<
   }
 }
 
+// The previous step's GJK direction of the pair (wi, wj), looked up only
+// when the pair reaches GJK: wi's entry wk of the previous step first (a
+// particle's list keeps its order while its neighbours stay), then its other
+// entries, four loads in flight at a time.  Records: float4 {direction in the
+// world frame, neighbour id bits} in entry order; nb = {first entry, count}
+// of wi in the previous step.
+struct WarmLookup {
+  const float4* prev;
+  const uint2* prev_nb;
+  uint32_t wi, wj, wk;
+  GR_D bool fetch(V3<float>& w) const {
+    const uint2 nb = prev_nb[wi];
+    const uint32_t np = nb.y;
+    const float4* pr = prev + nb.x;
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool hit = false;
+    if (wk < np) {
+      r = pr[wk];
+      hit = __float_as_uint(r.w) == wj;
+    }
+    for (uint32_t k0 = 0; !hit && k0 < np; k0 += 4) {
+      float4 q[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+        q[u] = k0 + u < np ? pr[k0 + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+        if (!hit && k0 + u < np && __float_as_uint(q[u].w) == wj) {
+          r = q[u];
+          hit = true;
+        }
+    }
+    w = {r.x, r.y, r.z};
+    const float nn = dot(w, w);
+    return hit && nn > 0.f && nn < INFINITY;
+  }
+};
+
 // Narrow phase + Hooke force of entry (si, sj) -- sorted positions -- from
 // si's side (ComponentManagerCPU.cpp:384-419).  Poses come from the sorted
 // records; velocities, needed only for contacts, from the planes through the
@@ -67,31 +105,16 @@ __device__ __forceinline__ uint32_t pair_force(const DevScene& d, uint32_t si, u
   CInfo<T> ci;
   if constexpr (sizeof(T) == 4 && SV && WARM) {
     if (e != ~0u) {
-      // the previous step's direction of this (i, j): slot k of i's list
-      // first (lists keep their order while neighbours stay), then a scan
       const uint64_t KN = (uint64_t)d.K * n;
       const uint32_t cur = *d.ws_par;
-      const uint32_t wi = d.sidx[si], wj = d.sidx[sj], wk = e - d.entry_off[si];
-      const float4* prev = d.ws + (uint64_t)(1u - cur) * KN;
-      const uint32_t np = d.ws_n[(uint64_t)(1u - cur) * n + wi];
-      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-      bool hit = false;
-      if (wk < np) {
-        r = prev[(uint64_t)wk * n + wi];
-        hit = __float_as_uint(r.w) == wj;
-      }
-      for (uint32_t k = 0; !hit && k < np; ++k) {
-        r = prev[(uint64_t)k * n + wi];
-        hit = __float_as_uint(r.w) == wj;
-      }
-      const V3<T> wv = {r.x, r.y, r.z};
-      const T nn = dot(wv, wv);
-      const bool use = hit && nn > T(0) && nn < T(INFINITY);
+      const uint32_t wj = d.sidx[sj];
+      const WarmLookup wl{d.ws + (uint64_t)(1u - cur) * KN, d.ws_nb + (uint64_t)(1u - cur) * n, d.sidx[si], wj,
+                          e - d.entry_off[si]};
       V3<T> dir = {T(0), T(0), T(0)};
-      ci = closest_points<T, SV, SCREEN>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st,
-                                         use ? &wv : nullptr, &dir);
-      if (!st.deferred)
-        d.ws[(uint64_t)cur * KN + (uint64_t)wk * n + wi] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(wj));
+      ci = closest_points<T, SV, SCREEN>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st, wl, &dir);
+      // every decided entry writes its record (a zero direction when GJK
+      // did not run), so an entry never holds a record of an earlier step
+      if (!st.deferred) d.ws[(uint64_t)cur * KN + e] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(wj));
     } else {
       ci = closest_points<T, SV, SCREEN>(A, B, a2w, b2w, args, vt_full<T>(d), vt_core<T>(d), st);
     }
@@ -250,7 +273,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DevScene d) {
     const PairRes<T>* res = (const PairRes<T>*)d.pair_res;
     const uint32_t s = d.rank[i];
     const uint32_t e0 = d.entry_off[s], e1 = d.entry_off[s + 1];
-    if (d.warm) d.ws_n[(uint64_t)(*d.ws_par) * n + i] = e1 - e0;  // the warm-start records of i this step
+    if (d.warm) d.ws_nb[(uint64_t)(*d.ws_par) * n + i] = make_uint2(e0, e1 - e0);  // i's warm-start records
     V3<double> F = {0.0, 0.0, 0.0}, M = {0.0, 0.0, 0.0};
     int32_t ncon = 0;
     for (uint32_t e = e0; e < e1; ++e) {

@@ -59,9 +59,9 @@ def test_passes_and_graphs_agree_bitwise():
         e.step(5, graph=graph)
         runs.append(e.get_state())
         e.close()
-    for r in runs[1:]:
+    for name, r in zip(("one pass", "two passes, graph"), runs[1:]):
         for a, b in zip(runs[0], r):
-            np.testing.assert_array_equal(a, b)
+            np.testing.assert_array_equal(a, b, err_msg=name)
 
 
 def test_new_state_starts_cold_and_iterations_drop():
