@@ -136,11 +136,21 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(uint64_t n, const uint
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  // the tile's keys and indices are loaded up front: the barriers of the
+  // item loop would otherwise wait on one load latency per item
+  uint32_t keys[kItems], idxs[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint64_t e = base + (uint64_t)k * kBlock + tid;
+    keys[k] = e < n ? kin[e] : 0u;
+    idxs[k] = e < n ? (identity_in ? (uint32_t)e : iin[e]) : 0u;
+  }
+#pragma unroll
   for (int k = 0; k < kItems; ++k) {
     const uint64_t e = base + (uint64_t)k * kBlock + tid;
     const bool valid = e < n;
-    uint32_t key = valid ? kin[e] : 0u;
-    uint32_t idx = valid ? (identity_in ? (uint32_t)e : iin[e]) : 0u;
+    const uint32_t key = keys[k];
+    const uint32_t idx = idxs[k];
     uint32_t dig = (key >> shift) & (D - 1u);
     // lanes of this wave holding the same digit
     uint64_t m = __ballot(valid);
