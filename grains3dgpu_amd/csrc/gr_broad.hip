@@ -612,9 +612,17 @@ __global__ void k_expand(uint64_t n, const uint32_t* __restrict__ cand, const ui
   if (s < n) {
     c = cnt[s];
     const uint32_t e0 = off[s];
-    for (uint32_t k = 0; k < c; ++k) {
-      pi[e0 + k] = (uint32_t)s;
-      pj[e0 + k] = cand[(uint64_t)k * n + s];
+    // four candidates loaded together: one load latency per four entries
+    for (uint32_t k0 = 0; k0 < c; k0 += 4) {
+      uint32_t v[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) v[u] = k0 + u < c ? cand[(uint64_t)(k0 + u) * n + s] : 0u;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+        if (k0 + u < c) {
+          pi[e0 + k0 + u] = (uint32_t)s;
+          pj[e0 + k0 + u] = v[u];
+        }
     }
   }
   count_add(counters, kCntCandidates, c);
