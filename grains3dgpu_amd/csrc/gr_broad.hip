@@ -295,13 +295,42 @@ __global__ void k_cell_off(const uint32_t* __restrict__ ks, const float4* __rest
                            uint64_t ncells, uint32_t* __restrict__ off, float4* __restrict__ sph) {
   const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c > ncells) return;
-  uint64_t lo = 0, hi = n;
+  // lower bound L = min{s : s == n or ks[s] >= c}, galloping from the
+  // previous step's offset of the cell (any start in [0, n] gives the same
+  // L; cells keep their offsets within a few positions from step to step,
+  // so a few dependent loads replace the ~20 of a full binary search)
+  auto at_or_after = [&](uint64_t t) { return t >= n || (uint64_t)ks[t] >= c; };
+  const uint64_t g = off[c] < n ? off[c] : n;
+  uint64_t lo, hi;
+  if (!at_or_after(g)) {  // L > g
+    lo = g + 1;
+    hi = n;
+    for (uint64_t step = 1; lo < n; step <<= 1) {
+      const uint64_t p = g + step < n ? g + step : n;
+      if (at_or_after(p)) {
+        hi = p;
+        break;
+      }
+      lo = p + 1;
+    }
+  } else {  // L <= g
+    lo = 0;
+    hi = g;
+    for (uint64_t step = 1; step <= g; step <<= 1) {
+      const uint64_t p = g - step;
+      if (!at_or_after(p)) {
+        lo = p + 1;
+        break;
+      }
+      hi = p;
+    }
+  }
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if ((uint64_t)ks[mid] < c)
-      lo = mid + 1;
-    else
+    if (at_or_after(mid))
       hi = mid;
+    else
+      lo = mid + 1;
   }
   off[c] = (uint32_t)lo;
   if (c == ncells || !sph) return;
