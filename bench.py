@@ -393,7 +393,7 @@ def bench_c1(args):
             "speedup_fp32_mixed_over_fp64": ps32 / (sc.n * K64 / r64["wall_s"])},
         fp32_mixed_cold_gjk={"particle_steps_per_s": sc.n * K / r32c["wall_s"],
                              "ms_per_step": r32c["wall_s"] * 1e3 / K,
-                             "gjk_iterations": r32c["stats"]["gjk_iterations"],
+                             "gjk_iterations_cold": r32c["stats"]["gjk_iterations"],
                              "gjk_iterations_warm": st["gjk_iterations"]},
         fp32_mixed_error_vs_fp64=err,
         fp32_mixed_error_vs_fp64_agitated=agit,

@@ -178,7 +178,8 @@ typedef struct grains_stats_t {
   uint64_t steps;
   uint64_t candidate_pairs;   /* ordered pairs passing the prefilter          */
   uint64_t gjk_calls;         /* ordered pairs that ran GJK                   */
-  uint64_t gjk_iterations;    /* summed GJK iterations                        */
+  uint64_t gjk_iterations;    /* summed GJK iterations (the support pair that
+                                 proves a separation is not counted)          */
   uint64_t contacts;          /* ordered particle-particle contacts           */
   uint64_t wall_contacts;     /* particle-obstacle contacts                   */
   uint64_t out_of_domain;     /* particle-steps whose centre left the grid    */
