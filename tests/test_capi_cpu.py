@@ -56,3 +56,17 @@ def test_create_without_device_fails_cleanly():
         pytest.skip("a device is present")
     with pytest.raises(engine.GrainsError):
         engine.Engine()
+
+
+def test_abi_constants_match_header():
+    """Every GRAINS_<NAME> integer define of the header that the Python
+    binding also names (<NAME>) has the same value there."""
+    import re
+    from grains3dgpu_amd import _abi as A
+    text = open(os.path.join(ROOT, "include", "grains_hip.h")).read()
+    found = 0
+    for name, val in re.findall(r"#define\s+GRAINS_(\w+)\s+\(?(-?0x[0-9a-fA-F]+|-?\d+)u?\)?", text):
+        if hasattr(A, name):
+            assert getattr(A, name) == int(val, 0), name
+            found += 1
+    assert found >= 10 and A.WARM_START == 0x4
